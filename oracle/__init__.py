@@ -1,0 +1,229 @@
+"""CPU oracle for the BioFVM diffusion-decay hot path (test infrastructure only).
+
+This package wraps ``liboracle.so`` (built from ``biofvm_oracle.c``), a plain C
+FP64 restatement of the reference's LOD_3D_BLOCKING step at world size 1
+(BioFVM/BioFVM_solvers.cpp:102-780), its Dirichlet pass
+(BioFVM/BioFVM_microenvironment.cpp:348-446) and the serial agent
+secretion/uptake loop (BioFVM/BioFVM_basic_agent.cpp:509-559).
+
+Only the tests, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` may use it, and only as the checker or the timed CPU baseline.
+The product path (``physicell-x_amd``) never imports it.
+
+Arrays are numpy float64 in the reference layout
+``idx(i,j,k,d) = ((i*ny + j)*nz + k)*ns + d`` (BioFVM/BioFVM_mesh.cpp:1157).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_up = ctypes.POINTER(ctypes.c_ubyte)
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE, "liboracle.so"],
+                       check=True, stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_num_threads.restype = ctypes.c_int
+        L.orc_set_num_threads.argtypes = [ctypes.c_int]
+        L.orc_mesh_nodes.restype = ctypes.c_int
+        L.orc_mesh_nodes.argtypes = [ctypes.c_double] * 3
+        L.orc_thomas_constants.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_double,
+                                           ctypes.c_double, _dp, _dp, _dp, _dp]
+        L.orc_thomas_axis.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_dirichlet_planes.argtypes = [ctypes.c_int] * 4 + [_up, _dp] + [_dp] * 6
+        L.orc_apply_dirichlet.argtypes = [_dp] + [ctypes.c_int] * 4 + [_up] + [_dp] * 6
+        L.orc_lod_step.argtypes = [_dp] + [ctypes.c_int] * 4 + [_dp] * 7 + [_up] + [_dp] * 6
+        L.orc_agent_constants.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_double] + [_dp] * 9
+        L.orc_secretion_uptake.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _ip, _up,
+                                           _dp, _dp, _dp, _dp, ctypes.c_double, _dp]
+        L.orc_total_mass.argtypes = [_dp, ctypes.c_size_t, ctypes.c_int, ctypes.c_double, _dp]
+        L.orc_voxel_indices.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, _ip]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def set_num_threads(n: int) -> None:
+    lib().orc_set_num_threads(int(n))
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+def mesh_nodes(start: float, end: float, dx: float) -> int:
+    return lib().orc_mesh_nodes(start, end, dx)
+
+
+def thomas_constants(D, decay, dt, dx):
+    D = _f64(D)
+    decay = _f64(decay)
+    ns = D.size
+    out = [np.zeros(ns) for _ in range(4)]
+    lib().orc_thomas_constants(ns, _d(D), _d(decay), dt, dx, *[_d(o) for o in out])
+    return tuple(out)  # c1, c2, c3, c3a
+
+
+def thomas_axis(n, consts):
+    c1, c2, c3, c3a = consts
+    ns = c1.size
+    denom = np.zeros((n, ns))
+    cp = np.zeros((n, ns))
+    lib().orc_thomas_axis(n, ns, _d(c1), _d(c2), _d(c3), _d(c3a), _d(denom), _d(cp))
+    return denom, cp
+
+
+def dirichlet_planes(nx, ny, nz, ns, face_registered, face_values):
+    reg = np.ascontiguousarray(face_registered, dtype=np.uint8)
+    vals = _f64(face_values).reshape(6, ns)
+    shapes = [(ny, nz), (ny, nz), (nx, nz), (nx, nz), (nx, ny), (nx, ny)]
+    planes = [np.zeros(s + (ns,)) for s in shapes]
+    lib().orc_dirichlet_planes(nx, ny, nz, ns, reg.ctypes.data_as(_up), _d(vals),
+                               *[_d(p) for p in planes])
+    return planes
+
+
+class Oracle:
+    """FP64 reference-order LOD solver state for one microenvironment.
+
+    Mirrors what ``Microenvironment`` caches for LOD_3D_BLOCKING: the solver
+    constants, the per-axis Thomas coefficients and the Dirichlet value table
+    (BioFVM/BioFVM_microenvironment.h:169-199, BioFVM/BioFVM_solvers.cpp:125-337).
+    """
+
+    def __init__(self, nx, ny, nz, ns, dx, D, decay, dirichlet_active=None,
+                 face_registered=None, face_values=None):
+        self.nx, self.ny, self.nz, self.ns = int(nx), int(ny), int(nz), int(ns)
+        self.dx = float(dx)
+        self.dV = self.dx ** 3
+        self.D = _f64(D).reshape(ns)
+        self.decay = _f64(decay).reshape(ns)
+        if dirichlet_active is None:
+            dirichlet_active = np.zeros(ns, np.uint8)
+        if face_registered is None:
+            face_registered = np.zeros(6, np.uint8)
+        if face_values is None:
+            face_values = np.ones((6, ns))
+        self.active = np.ascontiguousarray(dirichlet_active, dtype=np.uint8).reshape(ns)
+        self.face_registered = np.ascontiguousarray(face_registered, dtype=np.uint8).reshape(6)
+        self.face_values = _f64(face_values).reshape(6, ns)
+        self.planes = dirichlet_planes(nx, ny, nz, ns, self.face_registered, self.face_values)
+        self.dt = None
+
+    def precompute(self, dt):
+        if self.dt == dt:
+            return
+        self.consts = thomas_constants(self.D, self.decay, dt, self.dx)
+        self.tx = thomas_axis(self.nx, self.consts)
+        self.ty = thomas_axis(self.ny, self.consts)
+        self.tz = thomas_axis(self.nz, self.consts)
+        self.dt = dt
+
+    def check(self, u):
+        assert u.dtype == np.float64 and u.flags.c_contiguous
+        assert u.size == self.nx * self.ny * self.nz * self.ns
+
+    def apply_dirichlet(self, u):
+        self.check(u)
+        lib().orc_apply_dirichlet(_d(u), self.nx, self.ny, self.nz, self.ns,
+                                  self.active.ctypes.data_as(_up), *[_d(p) for p in self.planes])
+
+    def diffusion_step(self, u, dt):
+        """One simulate_diffusion_decay(dt) in place on the reference-layout field."""
+        self.check(u)
+        self.precompute(dt)
+        c1 = self.consts[0]
+        lib().orc_lod_step(_d(u), self.nx, self.ny, self.nz, self.ns, _d(c1),
+                           _d(self.tx[0]), _d(self.tx[1]), _d(self.ty[0]), _d(self.ty[1]),
+                           _d(self.tz[0]), _d(self.tz[1]), self.active.ctypes.data_as(_up),
+                           *[_d(p) for p in self.planes])
+
+    def secretion_uptake(self, u, agents, internalized=None):
+        """Serial simulate_secretion_and_uptake over ``agents`` (an AgentSet)."""
+        self.check(u)
+        n = agents.n
+        if n == 0:
+            return
+        ip = None
+        if internalized is not None:
+            assert internalized.dtype == np.float64 and internalized.shape == (n, self.ns)
+            ip = _d(internalized)
+        lib().orc_secretion_uptake(_d(u), self.ns, n, agents.voxel.ctypes.data_as(_ip),
+                                   agents.active.ctypes.data_as(_up), _d(agents.c1),
+                                   _d(agents.c2), _d(agents.e1), _d(agents.e2),
+                                   self.dV, ip)
+
+    def total_mass(self, u):
+        self.check(u)
+        out = np.zeros(self.ns)
+        lib().orc_total_mass(_d(u), self.nx * self.ny * self.nz, self.ns, self.dV, _d(out))
+        return out
+
+
+class AgentSet:
+    """Per-agent constants in agent-index order, as Basic_Agent holds them."""
+
+    def __init__(self, voxel, c1, c2, e1, e2, active=None):
+        self.voxel = np.ascontiguousarray(voxel, dtype=np.int32)
+        self.n = int(self.voxel.size)
+        ns = int(np.asarray(c1).size // max(self.n, 1)) if self.n else 0
+        shape = (self.n, ns)
+        self.c1 = _f64(c1).reshape(shape)
+        self.c2 = _f64(c2).reshape(shape)
+        self.e1 = _f64(e1).reshape(shape)
+        self.e2 = _f64(e2).reshape(shape)
+        if active is None:
+            active = np.ones(self.n, np.uint8)
+        self.active = np.ascontiguousarray(active, dtype=np.uint8)
+
+
+def agent_constants(dt, W, V, S, U, E, rho_star):
+    """Basic_Agent::set_internal_uptake_constants for all agents at once."""
+    V = _f64(V)
+    n = V.size
+    S, U, E, rho_star = (_f64(a).reshape(n, -1) for a in (S, U, E, rho_star))
+    ns = S.shape[1]
+    out = [np.zeros((n, ns)) for _ in range(4)]
+    lib().orc_agent_constants(n, ns, dt, W, _d(V), _d(S), _d(U), _d(E), _d(rho_star),
+                              *[_d(o) for o in out])
+    return tuple(out)  # c1, c2, e1, e2
+
+
+def voxel_indices(pos, bbox, dx, dy, dz, nx, ny, nz):
+    pos = _f64(pos).reshape(-1, 3)
+    bbox = _f64(bbox).reshape(6)
+    out = np.zeros(pos.shape[0], np.int32)
+    lib().orc_voxel_indices(pos.shape[0], _d(pos), _d(bbox), dx, dy, dz, nx, ny, nz,
+                            out.ctypes.data_as(_ip))
+    return out
