@@ -1,0 +1,228 @@
+"""Benchmark: BioFVM LOD diffusion-decay + agent secretion/uptake on one MI355X.
+
+Workload (default): 400^3 voxels x 4 substrates, dx = 20 um, dt = 0.01 min;
+D log-uniform in [1e3, 1e5] um^2/min, decay uniform in [0.01, 1] /min,
+Dirichlet value uniform in [0, 40] per substrate on all six faces, initial
+field = Dirichlet value + N(0,1); 100,000 agents at uniform random voxels with
+secretion and uptake rates uniform in [0, 10] /min, saturation density 40,
+volume 2494 um^3.  Synthetic data from a fixed seed.
+
+One "step" is one simulate_diffusion_decay(dt) followed by one
+simulate_cell_sources_and_sinks pass.  The headline value is the FP32
+scaled-deviation form in voxel*substrate updates per second; the FP64 path is
+measured beside it, and the FP32 result is compared with the FP64 result
+after the timed steps (the FP64 path itself is held to 1e-12 of the CPU
+oracle by the test-suite).
+
+The timed region runs K steps with one HIP event pair around every kernel on
+the library's stream; the per-kernel mean durations give the roofline figure
+of the dominant kernel.  The CPU baseline times the FP64 oracle (C/OpenMP) on
+the host on a bounded number of steps of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--no-cpu]
+Prints one JSON line.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "physicell-x_amd")
+PEAK_HBM_GBS = 8000.0
+METRIC = "voxel·substrate updates/s at 400³ (FP32 vs FP64); p50/p99 batched-solve latency"
+
+
+def load_pkg():
+    name = "physicellx_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    path = os.path.join(PKG_DIR, "__init__.py")
+    spec = importlib.util.spec_from_file_location(name, path, submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def agent_constants(dt, W, V, S, U, E, rho_star):
+    """Basic_Agent::set_internal_uptake_constants (BioFVM/BioFVM_basic_agent.cpp:224-274)
+    evaluated in numpy for the synthetic agents."""
+    f = (dt * V / W)[:, None]
+    c1 = S * rho_star * f
+    c2 = 1.0 + f * S + f * U
+    e2 = E * dt / W
+    return c1, c2, e2
+
+
+def make_workload(n, ns, n_agents, seed):
+    rng = np.random.default_rng(seed)
+    D = 10 ** rng.uniform(3, 5, ns)
+    decay = rng.uniform(0.01, 1.0, ns)
+    dval = rng.uniform(0, 40, ns)
+    # device (SoA) order: substrate-major blocks of n^3
+    field = np.empty(ns * n ** 3)
+    for d in range(ns):
+        blk = field[d * n ** 3:(d + 1) * n ** 3]
+        blk[:] = rng.standard_normal(n ** 3)
+        blk += dval[d]
+    voxel = rng.integers(0, n ** 3, n_agents).astype(np.int32)  # reference numbering
+    S = rng.uniform(0, 10, (n_agents, ns))
+    U = rng.uniform(0, 10, (n_agents, ns))
+    E = np.zeros((n_agents, ns))
+    rs = np.full((n_agents, ns), 40.0)
+    V = np.full(n_agents, 2494.0)
+    c1, c2, e2 = agent_constants(0.01, 20.0 ** 3, V, S, U, E, rs)
+    return dict(D=D, decay=decay, dval=dval, field=field, voxel=voxel, c1=c1, c2=c2, e2=e2,
+                S=S, U=U, E=E, rs=rs, V=V)
+
+
+def make_env(pkg, w, n, ns, precision):
+    env = pkg.Microenvironment(n, n, n, ns, 20.0, precision)
+    env.set_coefficients(w["D"], w["decay"])
+    env.set_dirichlet(np.ones(ns, np.uint8), np.ones(6, np.uint8), np.tile(w["dval"], (6, 1)))
+    env.upload(w["field"], "soa")
+    env.set_agents(w["voxel"], w["c1"], w["c2"], w["e2"])
+    return env
+
+
+def time_gpu(env, dt, steps, warmup):
+    env.step(dt, warmup)
+    env.synchronize()
+    t0 = time.perf_counter()
+    ms = env.time_kernels(dt, steps)  # K steps, HIP events around every kernel
+    env.synchronize()
+    wall = time.perf_counter() - t0
+    # graph-replayed steps, for reference
+    env.synchronize()
+    t1 = time.perf_counter()
+    env.step(dt, steps)
+    env.synchronize()
+    graph_wall = time.perf_counter() - t1
+    return wall, ms, graph_wall
+
+
+def load_traffic(kernel_key):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        doc = json.load(open(path))
+        return doc.get(kernel_key)
+    except Exception:
+        return None
+
+
+def cpu_baseline(w, n, ns, budget_s=15.0, max_steps=6):
+    sys.path.insert(0, ROOT)
+    import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    oracle.set_num_threads(threads)
+    ref_field = np.ascontiguousarray(w["field"].reshape(ns, n, n, n).transpose(3, 2, 1, 0)).ravel()
+    orc = oracle.Oracle(n, n, n, ns, 20.0, w["D"], w["decay"], np.ones(ns, np.uint8),
+                        np.ones(6, np.uint8), np.tile(w["dval"], (6, 1)))
+    W = 20.0 ** 3
+    c1, c2, e1, e2 = oracle.agent_constants(0.01, W, w["V"], w["S"], w["U"], w["E"], w["rs"])
+    ag = oracle.AgentSet(w["voxel"], c1, c2, e1, e2)
+    orc.precompute(0.01)
+    steps = 0
+    t0 = time.perf_counter()
+    while steps < max_steps:
+        orc.diffusion_step(ref_field, 0.01)
+        orc.secretion_uptake(ref_field, ag)
+        steps += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    return dict(value=n ** 3 * ns * steps / el, unit="voxel·substrate updates/s", cores=threads,
+                kind="port",
+                sample=f"FP64 oracle (C/OpenMP, reference loop order) on the {n}^3 x {ns} workload, "
+                       f"{steps} steps after the Thomas precompute, {el:.1f} s, {threads} threads")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=400)
+    ap.add_argument("--ns", type=int, default=4)
+    ap.add_argument("--agents", type=int, default=100_000)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the domain is resident on a single MI355X)", file=sys.stderr)
+        sys.exit(2)
+    pkg = load_pkg()
+    n, ns = args.n, args.ns
+    dt = 0.01
+    w = make_workload(n, ns, args.agents, args.seed)
+    updates = n ** 3 * ns
+
+    env32 = make_env(pkg, w, n, ns, "fp32")
+    wall32, ms32, gwall32 = time_gpu(env32, dt, args.steps, args.warmup)
+    tiles32 = env32.tiles()
+    res = {}
+    if not args.no_fp64:
+        env64 = make_env(pkg, w, n, ns, "fp64")
+        wall64, ms64, gwall64 = time_gpu(env64, dt, args.steps, args.warmup)
+        # both have now run warmup + 2*steps steps from the same start
+        f32 = env32.download("soa").reshape(ns, -1)
+        f64 = env64.download("soa").reshape(ns, -1)
+        m32, m64 = env32.total_mass(), env64.total_mass()
+        errs = [float(np.max(np.abs(f32[d] - f64[d])) / np.max(np.abs(f64[d]))) for d in range(ns)]
+        drift = [float(abs(m32[d] - m64[d]) / abs(m64[d])) for d in range(ns)]
+        res["fp64"] = dict(value=updates * args.steps / wall64, ms_per_step=1e3 * wall64 / args.steps,
+                           graph_ms_per_step=1e3 * gwall64 / args.steps,
+                           kernel_ms=dict(x=ms64[0], y=ms64[1], z=ms64[2], agents=ms64[3]))
+        res["fp32_vs_fp64"] = dict(max_rel_err_per_substrate=errs, mass_drift_per_substrate=drift,
+                                   steps=args.warmup + 2 * args.steps,
+                                   norm="max|f32 - f64| / max|f64| per substrate")
+        env64.close()
+    env32.close()
+
+    names = ["x", "y", "z"]
+    dom = int(np.argmax(ms32[:3]))
+    alg_bytes = updates * 2 * 4  # one FP32 read + one write per voxel-substrate per sweep
+    achieved = alg_bytes / (ms32[dom] * 1e-3) / 1e9
+    traffic = load_traffic(f"sweep_{names[dom]}_fp32_{n}")
+    out = {
+        "metric": METRIC,
+        "value": updates * args.steps / wall32,
+        "unit": "voxel·substrate updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall32 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{n}^3 voxels x {ns} substrates, {args.agents} agents, dt=0.01 min, "
+                               "FP32 scaled-deviation LOD step + agents",
+                   "grid": [n, n, n], "substrates": ns, "agents": args.agents, "seed": args.seed},
+        "roofline": {"bound": "hbm", "kernel": f"sweep_{names[dom]}", "achieved": achieved,
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": traffic},
+        "kernel_ms": dict(x=ms32[0], y=ms32[1], z=ms32[2], agents=ms32[3]),
+        "step_alg_frac_of_peak": (updates * 6 * 4) / (1e-3 * wall32 / args.steps) / 1e9 / PEAK_HBM_GBS,
+        "graph_ms_per_step": 1e3 * gwall32 / args.steps,
+        "tiles": dict(Lx=tiles32[0], Ly=tiles32[1], Lz=tiles32[2], rows_x=tiles32[3]),
+    }
+    out.update(res)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, n, ns)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

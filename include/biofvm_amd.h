@@ -1,0 +1,146 @@
+/*
+ * biofvm_amd.h -- C ABI of the MI355X (gfx950) BioFVM diffusion-decay library.
+ *
+ * One opaque handle (bfa_env) is one microenvironment: a uniform Cartesian
+ * grid of nx*ny*nz voxels of edge dx carrying ns substrates, resident on one
+ * GPU.  The handle replaces what the reference's LOD_3D_BLOCKING solver reads
+ * and writes inside Microenvironment, and the per-agent secretion/uptake loop:
+ *
+ *   reference entry point                                      replaced by
+ *   -----------------------------------------------------------------------------
+ *   Microenvironment::simulate_diffusion_decay(dt[,world,cart])  bfa_diffusion_decay
+ *     (BioFVM/BioFVM_microenvironment.cpp:1132-1162) dispatching to
+ *     diffusion_decay_solver__constant_coefficients_LOD_3D_BLOCKING
+ *     (BioFVM/BioFVM_solvers.cpp:102-780) through the function pointers
+ *     diffusion_decay_solver / diffusion_decay_solver_mpi
+ *     (BioFVM/BioFVM_microenvironment.h:199,206)
+ *   Microenvironment::simulate_cell_sources_and_sinks(dt)       bfa_secretion_uptake
+ *     (BioFVM/BioFVM_microenvironment.cpp:1370-1384) looping
+ *     Basic_Agent::simulate_secretion_and_uptake
+ *     (BioFVM/BioFVM_basic_agent.cpp:509-559)
+ *   main-loop pair main.cpp:244 then main.cpp:259                bfa_step
+ *   Microenvironment::apply_dirichlet_boundaries_conditions       bfa_apply_dirichlet
+ *     (BioFVM/BioFVM_microenvironment.cpp:348-446)
+ *
+ * Every function returns BFA_OK (0) or a nonzero bfa_status and never aborts;
+ * bfa_last_error() returns the message of the last failure on that handle.
+ * The reference's solver prints and returns on error
+ * (BioFVM/BioFVM_solvers.cpp:116-121); the host adapters map a nonzero status
+ * to the same print-and-return.  A handle is used by one host thread at a
+ * time; all its GPU work runs in order on the handle's own HIP stream.
+ */
+#ifndef BIOFVM_AMD_H
+#define BIOFVM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bfa_env bfa_env;
+
+enum bfa_status {
+    BFA_OK = 0,
+    BFA_ERR_ARG = 1,    /* invalid argument or size                     */
+    BFA_ERR_HIP = 2,    /* a HIP runtime call or kernel launch failed   */
+    BFA_ERR_STATE = 3,  /* call made before the state it needs was set  */
+    BFA_ERR_NOMEM = 4   /* device allocation failed or budget exceeded  */
+};
+
+enum bfa_precision {
+    BFA_FP64 = 0,  /* field stored and swept in FP64                          */
+    BFA_FP32 = 1   /* field stored and swept in FP32 as scaled deviations
+                      u = b + s*v; Thomas precompute and reductions in FP64  */
+};
+
+enum bfa_layout {
+    BFA_LAYOUT_REF = 0, /* reference order ((i*ny + j)*nz + k)*ns + d, i.e. the
+                           p_density_vectors layout of
+                           BioFVM/BioFVM_microenvironment.h:169 and
+                           BioFVM/BioFVM_mesh.cpp:1157                          */
+    BFA_LAYOUT_SOA = 1  /* device order ((d*nz + k)*ny + j)*nx + i            */
+};
+
+/* Allocates the device field (zeroed).  Replaces Microenvironment::resize_space
+ * (BioFVM/BioFVM_microenvironment.cpp:610-637) for the solver's storage.
+ * Edges up to 1024 voxels per axis; ns up to 64. */
+int bfa_create(bfa_env **out, int nx, int ny, int nz, int ns, double dx, int precision);
+void bfa_destroy(bfa_env *env);
+/* Message of the last failure on env; with env == NULL, of the last failed
+ * bfa_create in this thread. */
+const char *bfa_last_error(const bfa_env *env);
+
+/* Diffusion coefficients D[ns] (length^2/time) and decay rates decay[ns]
+ * (1/time): Microenvironment::diffusion_coefficients / decay_rates
+ * (BioFVM/BioFVM_microenvironment.h:183-184).  Invalidates the Thomas
+ * precompute. */
+int bfa_set_coefficients(bfa_env *env, const double *D, const double *decay);
+
+/* Dirichlet state.  active[ns] is dirichlet_activation_vector
+ * (BioFVM/BioFVM_microenvironment.cpp:265-273).  face_registered[6] and
+ * face_values[6*ns] (face-major, order xmin, xmax, ymin, ymax, zmin, zmax)
+ * describe the add_dirichlet_node calls made by initialize_microenvironment
+ * (BioFVM/BioFVM_microenvironment.cpp:2649-2786): a boundary voxel takes the
+ * values of the last registered face containing it, else 1.0 (:632). */
+int bfa_set_dirichlet(bfa_env *env, const uint8_t *active, const uint8_t *face_registered,
+                      const double *face_values);
+
+/* Host <-> device field transfer, FP64 on the host in the given layout.  For
+ * BFA_FP32 an upload also picks the deviation basis per substrate (b = the
+ * Dirichlet value when active, else the field mean; s = a power of two near
+ * max|u-b|) unless bfa_set_fp32_basis fixed one. */
+int bfa_upload(bfa_env *env, const double *field, int layout);
+int bfa_download(bfa_env *env, double *field, int layout);
+/* Fix the FP32 basis b[ns], s[ns] (s != 0) before the next upload. */
+int bfa_set_fp32_basis(bfa_env *env, const double *b, const double *s);
+int bfa_get_fp32_basis(const bfa_env *env, double *b, double *s);
+
+/* Agents in the order the reference's loop visits them (array order).
+ * voxel[n] is Basic_Agent::current_voxel_index in the reference numbering
+ * (BioFVM/BioFVM_mesh.cpp:1157); agents with active[a] == 0 or voxel < 0 are
+ * skipped (BioFVM/BioFVM_basic_agent.cpp:511-512).  c1, c2, e2 are [n][ns]:
+ * cell_source_sink_solver_temp1, _temp2 and _temp_export2 as set by
+ * Basic_Agent::set_internal_uptake_constants (BioFVM/BioFVM_basic_agent.cpp:
+ * 224-274).  n == 0 clears the agents. */
+int bfa_set_agents(bfa_env *env, int n, const int32_t *voxel, const uint8_t *active,
+                   const double *c1, const double *c2, const double *e2);
+
+/* One simulate_diffusion_decay(dt): Dirichlet, x sweep, Dirichlet, y sweep,
+ * Dirichlet, z sweep, Dirichlet (BioFVM/BioFVM_solvers.cpp:345-778). */
+int bfa_diffusion_decay(bfa_env *env, double dt);
+/* One simulate_cell_sources_and_sinks pass with the constants of
+ * bfa_set_agents, composed per voxel in ascending agent order. */
+int bfa_secretion_uptake(bfa_env *env);
+/* nsteps x (bfa_diffusion_decay(dt); bfa_secretion_uptake), replayed from a
+ * captured HIP graph. */
+int bfa_step(bfa_env *env, double dt, int nsteps);
+/* apply_dirichlet_boundaries_conditions on the resident field. */
+int bfa_apply_dirichlet(bfa_env *env);
+
+/* Total substrate per species, sum of rho*dx^3 over voxels, FP64 [ns]. */
+int bfa_total_mass(bfa_env *env, double *out);
+/* Wait for all work queued on the handle's stream. */
+int bfa_synchronize(bfa_env *env);
+
+/* Measurement helpers.  bfa_time_kernels runs `reps` diffusion+agent steps
+ * with one HIP event pair around every kernel on the handle's stream and
+ * returns the mean milliseconds per launch in ms[5] = {x sweep, y sweep,
+ * z sweep, agents, whole step}.  bfa_copy_kernel times a plain device copy of
+ * `bytes` bytes `reps` times (for calibrating HBM byte counters) and returns
+ * the mean milliseconds per launch. */
+int bfa_time_kernels(bfa_env *env, double dt, int reps, double *ms);
+int bfa_copy_kernel(size_t bytes, int reps, double *ms);
+
+/* Kernel shape selection (the chunk length L of the in-register line solve
+ * per axis, 0 = automatic; the x sweep's rows per LDS tile, 0 = automatic).
+ * Used by the autotuner. */
+int bfa_set_tiles(bfa_env *env, int Lx, int Ly, int Lz, int rows_x);
+int bfa_get_tiles(const bfa_env *env, int *Lx, int *Ly, int *Lz, int *rows_x);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BIOFVM_AMD_H */

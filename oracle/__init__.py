@@ -212,6 +212,9 @@ def agent_constants(dt, W, V, S, U, E, rho_star):
     """Basic_Agent::set_internal_uptake_constants for all agents at once."""
     V = _f64(V)
     n = V.size
+    if n == 0:
+        ns = np.asarray(S).shape[-1] if np.asarray(S).ndim == 2 else 0
+        return tuple(np.zeros((0, ns)) for _ in range(4))
     S, U, E, rho_star = (_f64(a).reshape(n, -1) for a in (S, U, E, rho_star))
     ns = S.shape[1]
     out = [np.zeros((n, ns)) for _ in range(4)]

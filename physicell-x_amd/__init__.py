@@ -1,0 +1,257 @@
+"""MI355X (gfx950) BioFVM diffusion-decay: Python host side over the C ABI.
+
+``Microenvironment`` mirrors the part of the reference's
+``BioFVM::Microenvironment`` (BioFVM/BioFVM_microenvironment.h:71-373) that the
+LOD diffusion-decay path touches: resize_space, diffusion_coefficients /
+decay_rates, Dirichlet activation and face values, density access,
+simulate_diffusion_decay(dt), simulate_cell_sources_and_sinks(dt) and
+apply_dirichlet_boundaries_conditions.  All compute runs in the HIP library
+``lib/libbiofvm_amd.so`` built from ``csrc/``; there is no CPU fallback, and a
+missing library raises at first use.
+
+Field arrays on the host are float64 in the reference layout
+``((i*ny + j)*nz + k)*ns + d`` unless ``layout="soa"`` is passed
+(``((d*nz + k)*ny + j)*nx + i``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libbiofvm_amd.so")
+
+FP64, FP32 = 0, 1
+LAYOUT_REF, LAYOUT_SOA = 0, 1
+_PREC = {"fp64": FP64, "f64": FP64, "double": FP64, FP64: FP64,
+         "fp32": FP32, "f32": FP32, "float": FP32, FP32: FP32}
+_LAYOUT = {"ref": LAYOUT_REF, "soa": LAYOUT_SOA, LAYOUT_REF: LAYOUT_REF, LAYOUT_SOA: LAYOUT_SOA}
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_u8 = ctypes.POINTER(ctypes.c_uint8)
+_i32 = ctypes.POINTER(ctypes.c_int32)
+_vp = ctypes.c_void_p
+
+
+class BioFVMError(RuntimeError):
+    pass
+
+
+def build(force: bool = False) -> str:
+    """Compile lib/libbiofvm_amd.so for gfx950 with hipcc (cross-compiles without a GPU)."""
+    cmd = ["make", "-C", HERE]
+    if force:
+        cmd.insert(1, "-B")
+    subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL)
+    return LIB_PATH
+
+
+def lib():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise BioFVMError(f"{LIB_PATH} is missing: build it first (make -C {HERE})")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER(ctypes.c_void_p)
+    L.bfa_create.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                             ctypes.c_double, ctypes.c_int]
+    L.bfa_destroy.argtypes = [_vp]
+    L.bfa_destroy.restype = None
+    L.bfa_last_error.argtypes = [_vp]
+    L.bfa_last_error.restype = ctypes.c_char_p
+    L.bfa_set_coefficients.argtypes = [_vp, _dp, _dp]
+    L.bfa_set_dirichlet.argtypes = [_vp, _u8, _u8, _dp]
+    L.bfa_upload.argtypes = [_vp, _dp, ctypes.c_int]
+    L.bfa_download.argtypes = [_vp, _dp, ctypes.c_int]
+    L.bfa_set_fp32_basis.argtypes = [_vp, _dp, _dp]
+    L.bfa_get_fp32_basis.argtypes = [_vp, _dp, _dp]
+    L.bfa_set_agents.argtypes = [_vp, ctypes.c_int, _i32, _u8, _dp, _dp, _dp]
+    L.bfa_diffusion_decay.argtypes = [_vp, ctypes.c_double]
+    L.bfa_secretion_uptake.argtypes = [_vp]
+    L.bfa_step.argtypes = [_vp, ctypes.c_double, ctypes.c_int]
+    L.bfa_apply_dirichlet.argtypes = [_vp]
+    L.bfa_total_mass.argtypes = [_vp, _dp]
+    L.bfa_synchronize.argtypes = [_vp]
+    L.bfa_time_kernels.argtypes = [_vp, ctypes.c_double, ctypes.c_int, _dp]
+    L.bfa_copy_kernel.argtypes = [ctypes.c_size_t, ctypes.c_int, _dp]
+    L.bfa_set_tiles.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.bfa_get_tiles.argtypes = [_vp] + [ctypes.POINTER(ctypes.c_int)] * 4
+    _lib = L
+    return L
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None and a.size != n:
+        raise BioFVMError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def copy_kernel_ms(nbytes: int, reps: int = 20) -> float:
+    """Mean time of a plain device copy of nbytes (HBM counter calibration)."""
+    out = ctypes.c_double()
+    rc = lib().bfa_copy_kernel(int(nbytes), int(reps), ctypes.byref(out))
+    if rc:
+        raise BioFVMError(f"bfa_copy_kernel failed with status {rc}")
+    return out.value
+
+
+class Microenvironment:
+    """One microenvironment resident on the current GPU.
+
+    ``Microenvironment(nx, ny, nz, ns, dx, precision)`` replaces
+    ``resize_space`` (BioFVM/BioFVM_microenvironment.cpp:610-637) for a
+    uniform mesh; the field starts at zero.
+    """
+
+    def __init__(self, nx, ny, nz, ns, dx, precision="fp64"):
+        self._h = None
+        L = lib()
+        h = ctypes.c_void_p()
+        rc = L.bfa_create(ctypes.byref(h), int(nx), int(ny), int(nz), int(ns), float(dx),
+                          _PREC[precision])
+        if rc:
+            raise BioFVMError(f"bfa_create: {L.bfa_last_error(None).decode()}")
+        self._h = h
+        self.nx, self.ny, self.nz, self.ns = int(nx), int(ny), int(nz), int(ns)
+        self.dx = float(dx)
+        self.precision = _PREC[precision]
+        self.diffusion_coefficients = np.zeros(self.ns)
+        self.decay_rates = np.zeros(self.ns)
+        self.dirichlet_activation_vector = np.zeros(self.ns, np.uint8)
+        self.face_registered = np.zeros(6, np.uint8)
+        self.face_values = np.ones((6, self.ns))
+
+    # -- lifecycle ------------------------------------------------------------
+    def close(self):
+        if self._h is not None:
+            lib().bfa_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc:
+            msg = lib().bfa_last_error(self._h).decode()
+            raise BioFVMError(f"{what}: status {rc}: {msg}")
+
+    @property
+    def number_of_voxels(self):
+        return self.nx * self.ny * self.nz
+
+    def number_of_densities(self):
+        return self.ns
+
+    # -- configuration ----------------------------------------------------------
+    def set_coefficients(self, D, decay):
+        self.diffusion_coefficients = _f64(D, self.ns).copy()
+        self.decay_rates = _f64(decay, self.ns).copy()
+        self._check(lib().bfa_set_coefficients(self._h, _d(self.diffusion_coefficients),
+                                               _d(self.decay_rates)), "set_coefficients")
+
+    def set_dirichlet(self, active, face_registered, face_values):
+        """dirichlet_activation_vector[ns], faces registered [6] (xmin..zmax) and
+        their value vectors [6][ns] as add_dirichlet_node would store them."""
+        self.dirichlet_activation_vector = np.ascontiguousarray(active, np.uint8).reshape(self.ns)
+        self.face_registered = np.ascontiguousarray(face_registered, np.uint8).reshape(6)
+        self.face_values = _f64(face_values, 6 * self.ns).reshape(6, self.ns).copy()
+        self._check(lib().bfa_set_dirichlet(self._h,
+                                            self.dirichlet_activation_vector.ctypes.data_as(_u8),
+                                            self.face_registered.ctypes.data_as(_u8),
+                                            _d(self.face_values)), "set_dirichlet")
+
+    def set_substrate_dirichlet_activation(self, index, flag):
+        """Microenvironment::set_substrate_dirichlet_activation
+        (BioFVM/BioFVM_microenvironment.cpp:265-273)."""
+        act = self.dirichlet_activation_vector.copy()
+        act[index] = 1 if flag else 0
+        self.set_dirichlet(act, self.face_registered, self.face_values)
+
+    def set_fp32_basis(self, b, s):
+        self._check(lib().bfa_set_fp32_basis(self._h, _d(_f64(b, self.ns)), _d(_f64(s, self.ns))),
+                    "set_fp32_basis")
+
+    def fp32_basis(self):
+        b = np.zeros(self.ns)
+        s = np.zeros(self.ns)
+        self._check(lib().bfa_get_fp32_basis(self._h, _d(b), _d(s)), "get_fp32_basis")
+        return b, s
+
+    def set_tiles(self, Lx=0, Ly=0, Lz=0, rows_x=0):
+        self._check(lib().bfa_set_tiles(self._h, int(Lx), int(Ly), int(Lz), int(rows_x)), "set_tiles")
+
+    def tiles(self):
+        v = [ctypes.c_int() for _ in range(4)]
+        self._check(lib().bfa_get_tiles(self._h, *[ctypes.byref(x) for x in v]), "get_tiles")
+        return tuple(x.value for x in v)
+
+    # -- field transfer -----------------------------------------------------------
+    def upload(self, field, layout="ref"):
+        f = _f64(field, self.number_of_voxels * self.ns)
+        self._check(lib().bfa_upload(self._h, _d(f), _LAYOUT[layout]), "upload")
+
+    def download(self, layout="ref"):
+        out = np.empty(self.number_of_voxels * self.ns)
+        self._check(lib().bfa_download(self._h, _d(out), _LAYOUT[layout]), "download")
+        return out
+
+    # -- agents --------------------------------------------------------------------
+    def set_agents(self, voxel, c1, c2, e2, active=None):
+        """Per-agent constants in agent order (reference voxel numbering)."""
+        voxel = np.ascontiguousarray(voxel, np.int32)
+        n = voxel.size
+        c1 = _f64(c1, n * self.ns)
+        c2 = _f64(c2, n * self.ns)
+        e2 = _f64(e2, n * self.ns)
+        act = None if active is None else np.ascontiguousarray(active, np.uint8)
+        self._check(lib().bfa_set_agents(self._h, n, voxel.ctypes.data_as(_i32),
+                                         None if act is None else act.ctypes.data_as(_u8),
+                                         _d(c1), _d(c2), _d(e2)), "set_agents")
+
+    # -- stepping ------------------------------------------------------------------
+    def simulate_diffusion_decay(self, dt):
+        self._check(lib().bfa_diffusion_decay(self._h, float(dt)), "simulate_diffusion_decay")
+
+    def simulate_cell_sources_and_sinks(self, dt=None):
+        """Agent secretion/uptake with the constants given to set_agents (they
+        already carry dt, as Basic_Agent's cached constants do)."""
+        self._check(lib().bfa_secretion_uptake(self._h), "simulate_cell_sources_and_sinks")
+
+    def step(self, dt, nsteps=1):
+        self._check(lib().bfa_step(self._h, float(dt), int(nsteps)), "step")
+
+    def apply_dirichlet_boundaries_conditions(self):
+        self._check(lib().bfa_apply_dirichlet(self._h), "apply_dirichlet_boundaries_conditions")
+
+    def total_mass(self):
+        out = np.zeros(self.ns)
+        self._check(lib().bfa_total_mass(self._h, _d(out)), "total_mass")
+        return out
+
+    def synchronize(self):
+        self._check(lib().bfa_synchronize(self._h), "synchronize")
+
+    def time_kernels(self, dt, reps=10):
+        ms = np.zeros(5)
+        self._check(lib().bfa_time_kernels(self._h, float(dt), int(reps), _d(ms)), "time_kernels")
+        return ms

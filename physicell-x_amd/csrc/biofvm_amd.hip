@@ -1,0 +1,1046 @@
+// biofvm_amd.hip -- host side of the C ABI in include/biofvm_amd.h.
+//
+// One bfa_env owns one microenvironment resident on the current HIP device:
+// the SoA field, the per-axis Thomas coefficients (rebuilt on the device in
+// FP64 whenever dt, the coefficients or the FP32 basis change), the Dirichlet
+// face table, the agents sorted into per-voxel segments, a HIP stream and a
+// captured one-step HIP graph.  See biofvm_kernels.h for the sweep design.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "biofvm_amd.h"
+#include "biofvm_aux_kernels.h"
+#include "biofvm_kernels.h"
+
+namespace {
+
+thread_local std::string g_create_error;
+
+constexpr int kMaxEdge = 1024;
+constexpr int kMaxSubstrates = 64;
+constexpr int kLs[] = {4, 8, 16, 32, 64};
+
+}  // namespace
+
+struct bfa_env {
+    int nx = 0, ny = 0, nz = 0, ns = 0;
+    double dx = 0.0;
+    int prec = BFA_FP64;
+    long long N = 0;
+    size_t tsize = 8;
+    int device = 0;
+    int max_lds = 65536;
+    hipStream_t stream = nullptr;
+    void* u = nullptr;
+
+    std::vector<double> D, decay;
+    bool coef_set = false;
+    std::vector<uint8_t> active, reg;
+    std::vector<double> fvals;  // [6][ns]
+    std::vector<double> b, s;   // FP32 basis
+    bool basis_fixed = false;
+
+    // device coefficient storage, axis a in {0,1,2}
+    double* den64[3] = {nullptr, nullptr, nullptr};
+    double* cp64[3] = {nullptr, nullptr, nullptr};
+    void* invT[3] = {nullptr, nullptr, nullptr};
+    void* cpT[3] = {nullptr, nullptr, nullptr};
+    void* kT = nullptr;
+    void* betaT = nullptr;
+    void* dvalT = nullptr;  // [7][ns]
+    double* params = nullptr;  // D, decay, b, s, 1/s   [5][ns]
+    double* red_partial = nullptr;
+    double* red_out = nullptr;
+    int red_blocks = 0;
+    double pre_dt = -1.0;
+    bool pre_valid = false;
+
+    // agents (sorted by device voxel, stable in agent index)
+    int nag = 0, nseg = 0;
+    int* seg_off = nullptr;
+    long long* seg_vox = nullptr;
+    void* agA = nullptr;  // FP64: c1   FP32: A
+    void* agB = nullptr;  // FP64: c2   FP32: B
+    void* agC = nullptr;  // FP64: e2
+    std::vector<double> h_c1, h_c2, h_e2;  // sorted, [ns][nag], for FP32 rebuilds
+    bool ag_f32_valid = false;
+
+    // graph of one step
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    double graph_dt = -1.0;
+    bool graph_valid = false;
+
+    int Lreq[3] = {0, 0, 0};
+    int Rreq = 0;
+    int L[3] = {0, 0, 0};
+    int R = 0;
+
+    std::string err;
+};
+
+namespace {
+
+int fail(bfa_env* e, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (e)
+        e->err = buf;
+    else
+        g_create_error = buf;
+    return code;
+}
+
+#define HIP_TRY(e, call)                                                                     \
+    do {                                                                                     \
+        hipError_t _st = (call);                                                             \
+        if (_st != hipSuccess)                                                               \
+            return fail((e), BFA_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(_st)); \
+    } while (0)
+
+#define LAUNCH_CHECK(e, what)                                                                 \
+    do {                                                                                      \
+        hipError_t _st = hipGetLastError();                                                   \
+        if (_st != hipSuccess)                                                                \
+            return fail((e), BFA_ERR_HIP, "launch of %s failed: %s", what, hipGetErrorString(_st)); \
+    } while (0)
+
+template <typename P>
+int dalloc(bfa_env* e, P** p, size_t bytes) {
+    void* q = nullptr;
+    hipError_t st = hipMalloc(&q, bytes ? bytes : 16);
+    if (st != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(e, BFA_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(st));
+    }
+    *p = reinterpret_cast<P*>(q);
+    return BFA_OK;
+}
+
+template <typename P>
+void dfree(P*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+int axis_len(const bfa_env* e, int a) { return a == 0 ? e->nx : (a == 1 ? e->ny : e->nz); }
+
+// Smallest chunk length whose chunk count fits pmax.
+int pick_L(int n, int pmax) {
+    for (int L : kLs)
+        if ((n + L - 1) / L <= pmax) return L;
+    return 0;
+}
+
+size_t lds_col(const bfa_env* e, int n, int L) {
+    const int P = (n + L - 1) / L;
+    return (size_t)(2 * n + 64 * P + P) * e->tsize;
+}
+
+size_t lds_x(const bfa_env* e, int R, int L) {
+    const int P = (e->nx + L - 1) / L;
+    const int S = e->nx | 1;
+    return ((size_t)R * S + 2 * e->nx + (size_t)P * R + P) * e->tsize;
+}
+
+int choose_tiles(bfa_env* e) {
+    for (int a = 1; a < 3; ++a) {
+        const int n = axis_len(e, a);
+        int L = e->Lreq[a] ? e->Lreq[a] : pick_L(n, 16);
+        if (L == 0 || (n + L - 1) / L > 16)
+            return fail(e, BFA_ERR_ARG, "axis %d of length %d: chunk length %d gives more than 16 chunks",
+                        a, n, L);
+        if (std::find(std::begin(kLs), std::end(kLs), L) == std::end(kLs))
+            return fail(e, BFA_ERR_ARG, "unsupported chunk length %d", L);
+        if (lds_col(e, n, L) > (size_t)e->max_lds)
+            return fail(e, BFA_ERR_ARG, "axis %d: LDS need %zu exceeds %d", a, lds_col(e, n, L),
+                        e->max_lds);
+        e->L[a] = L;
+    }
+    // x sweep: rows per tile R (power of two), LDS budget 64 KiB unless asked.
+    const int nx = e->nx;
+    int R = e->Rreq;
+    if (R == 0) {
+        R = 64;
+        const size_t budget = std::min<size_t>(65536, (size_t)e->max_lds);
+        while (R > 1) {
+            const int L = e->Lreq[0] ? e->Lreq[0] : pick_L(nx, std::max(1, 1024 / R));
+            if (L && lds_x(e, R, L) <= budget && ((nx + L - 1) / L) * R <= 1024) break;
+            R >>= 1;
+        }
+    }
+    if (R < 1 || R > 64 || (R & (R - 1)))
+        return fail(e, BFA_ERR_ARG, "x rows per tile must be a power of two in [1,64], got %d", R);
+    int L = e->Lreq[0] ? e->Lreq[0] : pick_L(nx, std::max(1, 1024 / R));
+    if (L == 0 || std::find(std::begin(kLs), std::end(kLs), L) == std::end(kLs))
+        return fail(e, BFA_ERR_ARG, "x axis of length %d: no chunk length fits %d rows", nx, R);
+    const int P = (nx + L - 1) / L;
+    const int thr = ((P * R + 63) / 64) * 64;
+    if (thr > 1024)
+        return fail(e, BFA_ERR_ARG, "x tile %d rows x %d chunks exceeds 1024 threads", R, P);
+    if (lds_x(e, R, L) > (size_t)e->max_lds)
+        return fail(e, BFA_ERR_ARG, "x tile LDS %zu exceeds %d", lds_x(e, R, L), e->max_lds);
+    e->L[0] = L;
+    e->R = R;
+    return BFA_OK;
+}
+
+template <typename T>
+int raise_lds_limits(bfa_env* e);
+
+int apply_tiles(bfa_env* e) {
+    int rc = choose_tiles(e);
+    if (rc) return rc;
+    return e->prec == BFA_FP32 ? raise_lds_limits<float>(e) : raise_lds_limits<double>(e);
+}
+
+template <typename T>
+bfa::SweepArgs<T> sweep_args(const bfa_env* e, int axis) {
+    bfa::SweepArgs<T> a;
+    a.u = reinterpret_cast<T*>(e->u);
+    a.inv = reinterpret_cast<const T*>(e->invT[axis]);
+    a.cp = reinterpret_cast<const T*>(e->cpT[axis]);
+    a.k = reinterpret_cast<const T*>(e->kT);
+    a.beta = reinterpret_cast<const T*>(e->betaT);
+    a.dval = reinterpret_cast<const T*>(e->dvalT);
+    unsigned long long act = 0;
+    for (int d = 0; d < e->ns; ++d)
+        if (e->active[d]) act |= 1ull << d;
+    a.active = act;
+    unsigned reg = 0;
+    for (int f = 0; f < 6; ++f)
+        if (e->reg[f]) reg |= 1u << f;
+    a.reg = reg;
+    a.N = e->N;
+    a.nx = e->nx;
+    a.ny = e->ny;
+    a.nz = e->nz;
+    a.ns = e->ns;
+    return a;
+}
+
+template <typename T, int L>
+int launch_x_L(bfa_env* e) {
+    constexpr bool DEV = std::is_same<T, float>::value;
+    const int R = e->R;
+    int logR = 0;
+    while ((1 << logR) < R) ++logR;
+    const int P = (e->nx + L - 1) / L;
+    const int thr = ((P * R + 63) / 64) * 64;
+    const long long rows = (long long)e->ny * e->nz;
+    const size_t lds = lds_x(e, R, L);
+    dim3 grid((unsigned)((rows + R - 1) / R), (unsigned)e->ns);
+    auto fn = bfa::k_sweep_x<T, L, DEV>;
+    hipLaunchKernelGGL(fn, grid, dim3(thr), lds, e->stream, sweep_args<T>(e, 0), R, logR);
+    LAUNCH_CHECK(e, "x sweep");
+    return BFA_OK;
+}
+
+template <typename T, int L, int AXIS>
+int launch_col_L(bfa_env* e) {
+    constexpr bool DEV = std::is_same<T, float>::value;
+    const int n = axis_len(e, AXIS);
+    const int P = (n + L - 1) / L;
+    const long long ncols = AXIS == 1 ? (long long)e->nz * e->nx : (long long)e->ny * e->nx;
+    const size_t lds = lds_col(e, n, L);
+    dim3 grid((unsigned)((ncols + 63) / 64), (unsigned)e->ns);
+    auto fn = bfa::k_sweep_col<T, L, AXIS, DEV>;
+    hipLaunchKernelGGL(fn, grid, dim3(64 * P), lds, e->stream, sweep_args<T>(e, AXIS));
+    LAUNCH_CHECK(e, AXIS == 1 ? "y sweep" : "z sweep");
+    return BFA_OK;
+}
+
+// Kernels whose dynamic LDS exceeds 64 KiB need the per-function limit raised
+// once, outside any stream capture.
+template <typename T>
+int raise_lds_limits(bfa_env* e) {
+    constexpr bool DEV = std::is_same<T, float>::value;
+    const void* fx = nullptr;
+    const void* fy = nullptr;
+    const void* fz = nullptr;
+#define BFA_PICK(Lv)                                                 \
+    if (e->L[0] == Lv) fx = (const void*)bfa::k_sweep_x<T, Lv, DEV>;    \
+    if (e->L[1] == Lv) fy = (const void*)bfa::k_sweep_col<T, Lv, 1, DEV>; \
+    if (e->L[2] == Lv) fz = (const void*)bfa::k_sweep_col<T, Lv, 2, DEV>;
+    BFA_PICK(4) BFA_PICK(8) BFA_PICK(16) BFA_PICK(32) BFA_PICK(64)
+#undef BFA_PICK
+    const size_t need[3] = {lds_x(e, e->R, e->L[0]), lds_col(e, e->ny, e->L[1]),
+                            lds_col(e, e->nz, e->L[2])};
+    const void* fn[3] = {fx, fy, fz};
+    for (int a = 0; a < 3; ++a)
+        if (need[a] > 65536 && fn[a])
+            HIP_TRY(e, hipFuncSetAttribute(fn[a], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)need[a]));
+    return BFA_OK;
+}
+
+template <typename T>
+int launch_sweep(bfa_env* e, int axis) {
+    const int L = e->L[axis];
+    if (axis == 0) {
+        switch (L) {
+            case 4: return launch_x_L<T, 4>(e);
+            case 8: return launch_x_L<T, 8>(e);
+            case 16: return launch_x_L<T, 16>(e);
+            case 32: return launch_x_L<T, 32>(e);
+            case 64: return launch_x_L<T, 64>(e);
+        }
+    } else if (axis == 1) {
+        switch (L) {
+            case 4: return launch_col_L<T, 4, 1>(e);
+            case 8: return launch_col_L<T, 8, 1>(e);
+            case 16: return launch_col_L<T, 16, 1>(e);
+            case 32: return launch_col_L<T, 32, 1>(e);
+            case 64: return launch_col_L<T, 64, 1>(e);
+        }
+    } else {
+        switch (L) {
+            case 4: return launch_col_L<T, 4, 2>(e);
+            case 8: return launch_col_L<T, 8, 2>(e);
+            case 16: return launch_col_L<T, 16, 2>(e);
+            case 32: return launch_col_L<T, 32, 2>(e);
+            case 64: return launch_col_L<T, 64, 2>(e);
+        }
+    }
+    return fail(e, BFA_ERR_ARG, "no kernel for chunk length %d", L);
+}
+
+int launch_sweep_any(bfa_env* e, int axis) {
+    return e->prec == BFA_FP32 ? launch_sweep<float>(e, axis) : launch_sweep<double>(e, axis);
+}
+
+int launch_agents(bfa_env* e) {
+    if (e->nseg == 0) return BFA_OK;
+    const long long work = (long long)e->nseg * e->ns;
+    const unsigned blocks = (unsigned)((work + 255) / 256);
+    if (e->prec == BFA_FP32) {
+        hipLaunchKernelGGL(bfa::k_agents_f32, dim3(blocks), dim3(256), 0, e->stream,
+                           reinterpret_cast<float*>(e->u), e->N, e->ns, e->nseg, e->nag,
+                           e->seg_off, e->seg_vox, reinterpret_cast<const float*>(e->agA),
+                           reinterpret_cast<const float*>(e->agB));
+    } else {
+        hipLaunchKernelGGL(bfa::k_agents_f64, dim3(blocks), dim3(256), 0, e->stream,
+                           reinterpret_cast<double*>(e->u), e->N, e->ns, e->nseg, e->nag,
+                           e->seg_off, e->seg_vox, reinterpret_cast<const double*>(e->agA),
+                           reinterpret_cast<const double*>(e->agB),
+                           reinterpret_cast<const double*>(e->agC));
+    }
+    LAUNCH_CHECK(e, "agents");
+    return BFA_OK;
+}
+
+void invalidate_graph(bfa_env* e) {
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->gexec = nullptr;
+    e->graph = nullptr;
+    e->graph_valid = false;
+}
+
+// Upload params (D, decay, b, s, 1/s) and the Dirichlet table in T.
+template <typename T>
+int upload_tables(bfa_env* e) {
+    const int ns = e->ns;
+    std::vector<double> p(5 * (size_t)ns);
+    for (int d = 0; d < ns; ++d) {
+        p[d] = e->D[d];
+        p[ns + d] = e->decay[d];
+        p[2 * ns + d] = e->b[d];
+        p[3 * ns + d] = e->s[d];
+        p[4 * ns + d] = 1.0 / e->s[d];
+    }
+    HIP_TRY(e, hipMemcpyAsync(e->params, p.data(), p.size() * sizeof(double), hipMemcpyHostToDevice,
+                              e->stream));
+    std::vector<T> dv(7 * (size_t)ns);
+    for (int f = 0; f < 7; ++f)
+        for (int d = 0; d < ns; ++d) {
+            const double val = f < 6 ? e->fvals[(size_t)f * ns + d] : 1.0;
+            dv[(size_t)f * ns + d] = (T)((val - e->b[d]) / e->s[d]);
+        }
+    HIP_TRY(e, hipMemcpyAsync(e->dvalT, dv.data(), dv.size() * sizeof(T), hipMemcpyHostToDevice,
+                              e->stream));
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    return BFA_OK;
+}
+
+template <typename T>
+int run_precompute(bfa_env* e, double dt) {
+    int rc = upload_tables<T>(e);
+    if (rc) return rc;
+    for (int a = 0; a < 3; ++a) {
+        hipLaunchKernelGGL(bfa::k_thomas<T>, dim3((e->ns + 63) / 64), dim3(64), 0, e->stream,
+                           axis_len(e, a), e->ns, e->params, dt, e->dx, e->den64[a], e->cp64[a],
+                           reinterpret_cast<T*>(e->invT[a]), reinterpret_cast<T*>(e->cpT[a]),
+                           reinterpret_cast<T*>(e->kT), reinterpret_cast<T*>(e->betaT));
+        LAUNCH_CHECK(e, "thomas precompute");
+    }
+    return BFA_OK;
+}
+
+int rebuild_f32_agents(bfa_env* e);
+
+int ensure_precompute(bfa_env* e, double dt) {
+    if (!e->coef_set) return fail(e, BFA_ERR_STATE, "bfa_set_coefficients has not been called");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(e, BFA_ERR_ARG, "dt must be positive, got %g", dt);
+    if (e->pre_valid && e->pre_dt == dt) return BFA_OK;
+    invalidate_graph(e);
+    int rc = e->prec == BFA_FP32 ? run_precompute<float>(e, dt) : run_precompute<double>(e, dt);
+    if (rc) return rc;
+    e->pre_dt = dt;
+    e->pre_valid = true;
+    return BFA_OK;
+}
+
+int enqueue_step(bfa_env* e) {
+    for (int a = 0; a < 3; ++a) {
+        int rc = launch_sweep_any(e, a);
+        if (rc) return rc;
+    }
+    return launch_agents(e);
+}
+
+int enqueue_diffusion(bfa_env* e) {
+    for (int a = 0; a < 3; ++a) {
+        int rc = launch_sweep_any(e, a);
+        if (rc) return rc;
+    }
+    return BFA_OK;
+}
+
+int build_graph(bfa_env* e, double dt) {
+    invalidate_graph(e);
+    HIP_TRY(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step(e);
+    hipGraph_t g = nullptr;
+    hipError_t st = hipStreamEndCapture(e->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (st != hipSuccess) return fail(e, BFA_ERR_HIP, "graph capture failed: %s", hipGetErrorString(st));
+    e->graph = g;
+    st = hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0);
+    if (st != hipSuccess) {
+        invalidate_graph(e);
+        return fail(e, BFA_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(st));
+    }
+    e->graph_dt = dt;
+    e->graph_valid = true;
+    return BFA_OK;
+}
+
+// FP32 agent map v <- A v + B from the FP64 constants and the basis.
+int rebuild_f32_agents(bfa_env* e) {
+    if (e->prec != BFA_FP32 || e->nag == 0) {
+        e->ag_f32_valid = true;
+        return BFA_OK;
+    }
+    const size_t cnt = (size_t)e->ns * e->nag;
+    std::vector<float> A(cnt), B(cnt);
+    for (int d = 0; d < e->ns; ++d)
+        for (int a = 0; a < e->nag; ++a) {
+            const size_t o = (size_t)d * e->nag + a;
+            const double c1 = e->h_c1[o], c2 = e->h_c2[o], e2 = e->h_e2[o];
+            A[o] = (float)(1.0 / c2);
+            B[o] = (float)((((e->b[d] + c1) / c2) + e2 - e->b[d]) / e->s[d]);
+        }
+    HIP_TRY(e, hipMemcpy(e->agA, A.data(), cnt * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(e, hipMemcpy(e->agB, B.data(), cnt * sizeof(float), hipMemcpyHostToDevice));
+    e->ag_f32_valid = true;
+    return BFA_OK;
+}
+
+int reduce(bfa_env* e, const void* x, bool f32, long long count, long long stride, long long dstride,
+           int mode, const double* ref_dev, double* host_out) {
+    const int nb = e->red_blocks;
+    dim3 grid(nb, e->ns);
+    if (f32) {
+        if (mode == 0)
+            hipLaunchKernelGGL((bfa::k_reduce_partial<float, 0>), grid, dim3(256), 0, e->stream,
+                               (const float*)x, count, stride, dstride, ref_dev, e->red_partial);
+        else
+            hipLaunchKernelGGL((bfa::k_reduce_partial<float, 1>), grid, dim3(256), 0, e->stream,
+                               (const float*)x, count, stride, dstride, ref_dev, e->red_partial);
+    } else {
+        if (mode == 0)
+            hipLaunchKernelGGL((bfa::k_reduce_partial<double, 0>), grid, dim3(256), 0, e->stream,
+                               (const double*)x, count, stride, dstride, ref_dev, e->red_partial);
+        else
+            hipLaunchKernelGGL((bfa::k_reduce_partial<double, 1>), grid, dim3(256), 0, e->stream,
+                               (const double*)x, count, stride, dstride, ref_dev, e->red_partial);
+    }
+    LAUNCH_CHECK(e, "reduce partial");
+    if (mode == 0)
+        hipLaunchKernelGGL(bfa::k_reduce_final<0>, dim3(e->ns), dim3(64), 0, e->stream,
+                           e->red_partial, nb, e->red_out);
+    else
+        hipLaunchKernelGGL(bfa::k_reduce_final<1>, dim3(e->ns), dim3(64), 0, e->stream,
+                           e->red_partial, nb, e->red_out);
+    LAUNCH_CHECK(e, "reduce final");
+    HIP_TRY(e, hipMemcpyAsync(host_out, e->red_out, e->ns * sizeof(double), hipMemcpyDeviceToHost,
+                              e->stream));
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    return BFA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bfa_last_error(const bfa_env* env) {
+    return env ? env->err.c_str() : g_create_error.c_str();
+}
+
+int bfa_create(bfa_env** out, int nx, int ny, int nz, int ns, double dx, int precision) {
+    if (!out) return fail(nullptr, BFA_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (nx < 1 || ny < 1 || nz < 1 || nx > kMaxEdge || ny > kMaxEdge || nz > kMaxEdge)
+        return fail(nullptr, BFA_ERR_ARG, "grid %dx%dx%d outside [1,%d] per axis", nx, ny, nz, kMaxEdge);
+    if (ns < 1 || ns > kMaxSubstrates)
+        return fail(nullptr, BFA_ERR_ARG, "ns=%d outside [1,%d]", ns, kMaxSubstrates);
+    if (!(dx > 0.0) || !std::isfinite(dx)) return fail(nullptr, BFA_ERR_ARG, "dx must be positive");
+    if (precision != BFA_FP64 && precision != BFA_FP32)
+        return fail(nullptr, BFA_ERR_ARG, "unknown precision %d", precision);
+    bfa_env* e = new bfa_env();
+    e->nx = nx;
+    e->ny = ny;
+    e->nz = nz;
+    e->ns = ns;
+    e->dx = dx;
+    e->prec = precision;
+    e->N = (long long)nx * ny * nz;
+    e->tsize = precision == BFA_FP32 ? 4 : 8;
+    e->active.assign(ns, 0);
+    e->reg.assign(6, 0);
+    e->fvals.assign(6 * (size_t)ns, 1.0);
+    e->b.assign(ns, 0.0);
+    e->s.assign(ns, 1.0);
+    auto bail = [&](int rc) {
+        g_create_error = e->err;
+        bfa_destroy(e);
+        return rc;
+    };
+    hipError_t st = hipGetDevice(&e->device);
+    if (st != hipSuccess) {
+        fail(e, BFA_ERR_HIP, "hipGetDevice failed: %s", hipGetErrorString(st));
+        return bail(BFA_ERR_HIP);
+    }
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, e->device) == hipSuccess &&
+        lds > 0)
+        e->max_lds = lds;
+    st = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (st != hipSuccess) {
+        fail(e, BFA_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(st));
+        return bail(BFA_ERR_HIP);
+    }
+    int rc = dalloc(e, &e->u, (size_t)e->N * ns * e->tsize);
+    if (rc) return bail(rc);
+    st = hipMemsetAsync(e->u, 0, (size_t)e->N * ns * e->tsize, e->stream);
+    if (st != hipSuccess) {
+        fail(e, BFA_ERR_HIP, "hipMemset failed: %s", hipGetErrorString(st));
+        return bail(BFA_ERR_HIP);
+    }
+    for (int a = 0; a < 3; ++a) {
+        const size_t n = (size_t)axis_len(e, a) * ns;
+        if ((rc = dalloc(e, &e->den64[a], n * 8)) || (rc = dalloc(e, &e->cp64[a], n * 8)) ||
+            (rc = dalloc(e, &e->invT[a], n * e->tsize)) || (rc = dalloc(e, &e->cpT[a], n * e->tsize)))
+            return bail(rc);
+    }
+    if ((rc = dalloc(e, &e->kT, ns * e->tsize)) || (rc = dalloc(e, &e->betaT, ns * e->tsize)) ||
+        (rc = dalloc(e, &e->dvalT, 7 * (size_t)ns * e->tsize)) ||
+        (rc = dalloc(e, &e->params, 5 * (size_t)ns * 8)))
+        return bail(rc);
+    e->red_blocks = (int)std::min<long long>(512, std::max<long long>(1, e->N / 8192));
+    if ((rc = dalloc(e, &e->red_partial, (size_t)e->red_blocks * ns * 8)) ||
+        (rc = dalloc(e, &e->red_out, (size_t)ns * 8)))
+        return bail(rc);
+    if ((rc = apply_tiles(e))) return bail(rc);
+    st = hipStreamSynchronize(e->stream);
+    if (st != hipSuccess) {
+        fail(e, BFA_ERR_HIP, "initial sync failed: %s", hipGetErrorString(st));
+        return bail(BFA_ERR_HIP);
+    }
+    *out = e;
+    return BFA_OK;
+}
+
+void bfa_destroy(bfa_env* e) {
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    invalidate_graph(e);
+    dfree(e->u);
+    for (int a = 0; a < 3; ++a) {
+        dfree(e->den64[a]);
+        dfree(e->cp64[a]);
+        dfree(e->invT[a]);
+        dfree(e->cpT[a]);
+    }
+    dfree(e->kT);
+    dfree(e->betaT);
+    dfree(e->dvalT);
+    dfree(e->params);
+    dfree(e->red_partial);
+    dfree(e->red_out);
+    dfree(e->seg_off);
+    dfree(e->seg_vox);
+    dfree(e->agA);
+    dfree(e->agB);
+    dfree(e->agC);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+int bfa_set_coefficients(bfa_env* e, const double* D, const double* decay) {
+    if (!e || !D || !decay) return fail(e, BFA_ERR_ARG, "NULL argument");
+    for (int d = 0; d < e->ns; ++d)
+        if (!std::isfinite(D[d]) || !std::isfinite(decay[d]) || D[d] < 0 || decay[d] < 0)
+            return fail(e, BFA_ERR_ARG, "substrate %d: D=%g decay=%g must be finite and >= 0", d, D[d],
+                        decay[d]);
+    e->D.assign(D, D + e->ns);
+    e->decay.assign(decay, decay + e->ns);
+    e->coef_set = true;
+    e->pre_valid = false;
+    invalidate_graph(e);
+    return BFA_OK;
+}
+
+int bfa_set_dirichlet(bfa_env* e, const uint8_t* active, const uint8_t* face_registered,
+                      const double* face_values) {
+    if (!e || !active || !face_registered || !face_values) return fail(e, BFA_ERR_ARG, "NULL argument");
+    e->active.assign(active, active + e->ns);
+    for (auto& a : e->active) a = a ? 1 : 0;
+    e->reg.assign(face_registered, face_registered + 6);
+    for (auto& r : e->reg) r = r ? 1 : 0;
+    e->fvals.assign(face_values, face_values + 6 * (size_t)e->ns);
+    e->pre_valid = false;
+    invalidate_graph(e);
+    return BFA_OK;
+}
+
+int bfa_set_fp32_basis(bfa_env* e, const double* b, const double* s) {
+    if (!e || !b || !s) return fail(e, BFA_ERR_ARG, "NULL argument");
+    for (int d = 0; d < e->ns; ++d)
+        if (!std::isfinite(b[d]) || !std::isfinite(s[d]) || s[d] == 0.0)
+            return fail(e, BFA_ERR_ARG, "substrate %d: basis b=%g s=%g invalid", d, b[d], s[d]);
+    e->b.assign(b, b + e->ns);
+    e->s.assign(s, s + e->ns);
+    e->basis_fixed = true;
+    return BFA_OK;
+}
+
+int bfa_get_fp32_basis(const bfa_env* e, double* b, double* s) {
+    if (!e || !b || !s) return BFA_ERR_ARG;
+    std::copy(e->b.begin(), e->b.end(), b);
+    std::copy(e->s.begin(), e->s.end(), s);
+    return BFA_OK;
+}
+
+int bfa_upload(bfa_env* e, const double* field, int layout) {
+    if (!e || !field) return fail(e, BFA_ERR_ARG, "NULL argument");
+    if (layout != BFA_LAYOUT_REF && layout != BFA_LAYOUT_SOA)
+        return fail(e, BFA_ERR_ARG, "unknown layout %d", layout);
+    const size_t total = (size_t)e->N * e->ns;
+    double* stage = nullptr;
+    const bool direct = (e->prec == BFA_FP64 && layout == BFA_LAYOUT_SOA);
+    if (direct) {
+        HIP_TRY(e, hipMemcpyAsync(e->u, field, total * 8, hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(e, hipStreamSynchronize(e->stream));
+        return BFA_OK;
+    }
+    int rc = dalloc(e, &stage, total * 8);
+    if (rc) return rc;
+    auto cleanup = [&](int code) {
+        (void)hipStreamSynchronize(e->stream);
+        dfree(stage);
+        return code;
+    };
+    hipError_t st = hipMemcpyAsync(stage, field, total * 8, hipMemcpyHostToDevice, e->stream);
+    if (st != hipSuccess)
+        return cleanup(fail(e, BFA_ERR_HIP, "upload copy failed: %s", hipGetErrorString(st)));
+    // basis for the FP32 deviation form
+    if (e->prec == BFA_FP32 && !e->basis_fixed) {
+        const long long stride = layout == BFA_LAYOUT_REF ? e->ns : 1;
+        const long long dstride = layout == BFA_LAYOUT_REF ? 1 : e->N;
+        std::vector<double> sum(e->ns), mx(e->ns);
+        if ((rc = reduce(e, stage, false, e->N, stride, dstride, 0, nullptr, sum.data())))
+            return cleanup(rc);
+        for (int d = 0; d < e->ns; ++d) {
+            double b = sum[d] / (double)e->N;
+            if (e->active[d]) {
+                b = 1.0;
+                for (int f = 5; f >= 0; --f)
+                    if (e->reg[f]) {
+                        b = e->fvals[(size_t)f * e->ns + d];
+                        break;
+                    }
+            }
+            e->b[d] = b;
+        }
+        std::vector<double> bdev(e->b);
+        double* bref = nullptr;
+        if ((rc = dalloc(e, &bref, e->ns * 8))) return cleanup(rc);
+        st = hipMemcpyAsync(bref, bdev.data(), e->ns * 8, hipMemcpyHostToDevice, e->stream);
+        if (st == hipSuccess) rc = reduce(e, stage, false, e->N, stride, dstride, 1, bref, mx.data());
+        (void)hipStreamSynchronize(e->stream);
+        dfree(bref);
+        if (st != hipSuccess) return cleanup(fail(e, BFA_ERR_HIP, "basis copy failed"));
+        if (rc) return cleanup(rc);
+        for (int d = 0; d < e->ns; ++d) {
+            int ex = 0;
+            if (mx[d] > 0.0 && std::isfinite(mx[d])) {
+                std::frexp(mx[d], &ex);
+                e->s[d] = std::ldexp(1.0, ex);
+            } else {
+                e->s[d] = 1.0;
+            }
+        }
+    }
+    if (e->prec == BFA_FP32) {
+        e->pre_valid = false;  // beta and the Dirichlet table depend on the basis
+        e->ag_f32_valid = false;
+    }
+    // params buffer: b at [2ns], 1/s at [4ns] (and s at [3ns]) for the conversion
+    std::vector<double> p(5 * (size_t)e->ns, 0.0);
+    for (int d = 0; d < e->ns; ++d) {
+        p[d] = e->coef_set ? e->D[d] : 0.0;
+        p[e->ns + d] = e->coef_set ? e->decay[d] : 0.0;
+        p[2 * e->ns + d] = e->b[d];
+        p[3 * e->ns + d] = e->s[d];
+        p[4 * e->ns + d] = 1.0 / e->s[d];
+    }
+    st = hipMemcpyAsync(e->params, p.data(), p.size() * 8, hipMemcpyHostToDevice, e->stream);
+    if (st != hipSuccess) return cleanup(fail(e, BFA_ERR_HIP, "params copy failed"));
+    const double* bd = e->params + 2 * e->ns;
+    const double* rsd = e->params + 4 * e->ns;
+    if (layout == BFA_LAYOUT_REF) {
+        dim3 grid((unsigned)(((long long)e->nz * e->ns + 31) / 32), (unsigned)((e->nx + 31) / 32),
+                  (unsigned)e->ny);
+        if (e->prec == BFA_FP32)
+            hipLaunchKernelGGL(bfa::k_ref_to_soa<float>, grid, dim3(32, 8), 0, e->stream, stage,
+                               (float*)e->u, e->nx, e->ny, e->nz, e->ns, bd, rsd);
+        else
+            hipLaunchKernelGGL(bfa::k_ref_to_soa<double>, grid, dim3(32, 8), 0, e->stream, stage,
+                               (double*)e->u, e->nx, e->ny, e->nz, e->ns, bd, rsd);
+    } else {
+        dim3 grid((unsigned)std::min<long long>(4096, (e->N + 255) / 256), (unsigned)e->ns);
+        hipLaunchKernelGGL(bfa::k_soa_from_f64<float>, grid, dim3(256), 0, e->stream, stage,
+                           (float*)e->u, e->N, bd, rsd);
+    }
+    st = hipGetLastError();
+    if (st != hipSuccess) return cleanup(fail(e, BFA_ERR_HIP, "upload conversion launch failed: %s",
+                                              hipGetErrorString(st)));
+    st = hipStreamSynchronize(e->stream);
+    if (st != hipSuccess) return cleanup(fail(e, BFA_ERR_HIP, "upload failed: %s", hipGetErrorString(st)));
+    return cleanup(BFA_OK);
+}
+
+int bfa_download(bfa_env* e, double* field, int layout) {
+    if (!e || !field) return fail(e, BFA_ERR_ARG, "NULL argument");
+    if (layout != BFA_LAYOUT_REF && layout != BFA_LAYOUT_SOA)
+        return fail(e, BFA_ERR_ARG, "unknown layout %d", layout);
+    const size_t total = (size_t)e->N * e->ns;
+    if (e->prec == BFA_FP64 && layout == BFA_LAYOUT_SOA) {
+        HIP_TRY(e, hipMemcpyAsync(field, e->u, total * 8, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(e, hipStreamSynchronize(e->stream));
+        return BFA_OK;
+    }
+    double* stage = nullptr;
+    int rc = dalloc(e, &stage, total * 8);
+    if (rc) return rc;
+    auto cleanup = [&](int code) {
+        (void)hipStreamSynchronize(e->stream);
+        dfree(stage);
+        return code;
+    };
+    // b and s for the back-conversion live in a small scratch copy
+    std::vector<double> bs(2 * (size_t)e->ns);
+    for (int d = 0; d < e->ns; ++d) {
+        bs[d] = e->b[d];
+        bs[e->ns + d] = e->s[d];
+    }
+    double* dbs = nullptr;
+    if ((rc = dalloc(e, &dbs, bs.size() * 8))) return cleanup(rc);
+    hipError_t st = hipMemcpyAsync(dbs, bs.data(), bs.size() * 8, hipMemcpyHostToDevice, e->stream);
+    if (st == hipSuccess) {
+        if (layout == BFA_LAYOUT_REF) {
+            dim3 grid((unsigned)(((long long)e->nz * e->ns + 31) / 32), (unsigned)((e->nx + 31) / 32),
+                      (unsigned)e->ny);
+            if (e->prec == BFA_FP32)
+                hipLaunchKernelGGL(bfa::k_soa_to_ref<float>, grid, dim3(32, 8), 0, e->stream,
+                                   (const float*)e->u, stage, e->nx, e->ny, e->nz, e->ns, dbs,
+                                   dbs + e->ns);
+            else
+                hipLaunchKernelGGL(bfa::k_soa_to_ref<double>, grid, dim3(32, 8), 0, e->stream,
+                                   (const double*)e->u, stage, e->nx, e->ny, e->nz, e->ns, dbs,
+                                   dbs + e->ns);
+        } else {
+            dim3 grid((unsigned)std::min<long long>(4096, (e->N + 255) / 256), (unsigned)e->ns);
+            hipLaunchKernelGGL(bfa::k_soa_to_f64<float>, grid, dim3(256), 0, e->stream,
+                               (const float*)e->u, stage, e->N, dbs, dbs + e->ns);
+        }
+        st = hipGetLastError();
+    }
+    if (st == hipSuccess) st = hipMemcpyAsync(field, stage, total * 8, hipMemcpyDeviceToHost, e->stream);
+    if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+    dfree(dbs);
+    if (st != hipSuccess) return cleanup(fail(e, BFA_ERR_HIP, "download failed: %s", hipGetErrorString(st)));
+    return cleanup(BFA_OK);
+}
+
+int bfa_set_agents(bfa_env* e, int n, const int32_t* voxel, const uint8_t* active, const double* c1,
+                   const double* c2, const double* e2) {
+    if (!e) return BFA_ERR_ARG;
+    if (n < 0) return fail(e, BFA_ERR_ARG, "negative agent count");
+    if (n > 0 && (!voxel || !c1 || !c2 || !e2)) return fail(e, BFA_ERR_ARG, "NULL argument");
+    invalidate_graph(e);
+    dfree(e->seg_off);
+    dfree(e->seg_vox);
+    dfree(e->agA);
+    dfree(e->agB);
+    dfree(e->agC);
+    e->nag = e->nseg = 0;
+    e->h_c1.clear();
+    e->h_c2.clear();
+    e->h_e2.clear();
+    if (n == 0) return BFA_OK;
+    const int ns = e->ns;
+    const long long nvox = e->N;
+    // keep active agents with a valid voxel, keyed by device voxel index
+    std::vector<long long> key;
+    std::vector<int> idx;
+    key.reserve(n);
+    idx.reserve(n);
+    for (int a = 0; a < n; ++a) {
+        if (active && !active[a]) continue;
+        const long long v = voxel[a];
+        if (v < 0) continue;
+        if (v >= nvox) return fail(e, BFA_ERR_ARG, "agent %d: voxel %lld out of range", a, v);
+        const long long kk = v % e->nz, jj = (v / e->nz) % e->ny, ii = v / ((long long)e->nz * e->ny);
+        key.push_back((kk * e->ny + jj) * e->nx + ii);
+        idx.push_back(a);
+    }
+    const int m = (int)idx.size();
+    if (m == 0) return BFA_OK;
+    std::vector<int> order(m);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key[x] < key[y]; });
+    std::vector<int> off;
+    std::vector<long long> svox;
+    for (int r = 0; r < m; ++r)
+        if (r == 0 || key[order[r]] != key[order[r - 1]]) {
+            off.push_back(r);
+            svox.push_back(key[order[r]]);
+        }
+    off.push_back(m);
+    e->h_c1.resize((size_t)ns * m);
+    e->h_c2.resize((size_t)ns * m);
+    e->h_e2.resize((size_t)ns * m);
+    for (int r = 0; r < m; ++r) {
+        const int a = idx[order[r]];
+        for (int d = 0; d < ns; ++d) {
+            const size_t src = (size_t)a * ns + d, dst = (size_t)d * m + r;
+            e->h_c1[dst] = c1[src];
+            e->h_c2[dst] = c2[src];
+            e->h_e2[dst] = e2[src];
+        }
+    }
+    const int nseg = (int)svox.size();
+    int rc;
+    if ((rc = dalloc(e, &e->seg_off, off.size() * sizeof(int))) ||
+        (rc = dalloc(e, &e->seg_vox, svox.size() * sizeof(long long))))
+        return rc;
+    HIP_TRY(e, hipMemcpy(e->seg_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(e, hipMemcpy(e->seg_vox, svox.data(), svox.size() * sizeof(long long), hipMemcpyHostToDevice));
+    const size_t cnt = (size_t)ns * m;
+    if (e->prec == BFA_FP64) {
+        if ((rc = dalloc(e, &e->agA, cnt * 8)) || (rc = dalloc(e, &e->agB, cnt * 8)) ||
+            (rc = dalloc(e, &e->agC, cnt * 8)))
+            return rc;
+        HIP_TRY(e, hipMemcpy(e->agA, e->h_c1.data(), cnt * 8, hipMemcpyHostToDevice));
+        HIP_TRY(e, hipMemcpy(e->agB, e->h_c2.data(), cnt * 8, hipMemcpyHostToDevice));
+        HIP_TRY(e, hipMemcpy(e->agC, e->h_e2.data(), cnt * 8, hipMemcpyHostToDevice));
+    } else {
+        if ((rc = dalloc(e, &e->agA, cnt * 4)) || (rc = dalloc(e, &e->agB, cnt * 4))) return rc;
+    }
+    e->nag = m;
+    e->nseg = nseg;
+    e->ag_f32_valid = false;
+    return rebuild_f32_agents(e);
+}
+
+static int ready(bfa_env* e, double dt) {
+    int rc = ensure_precompute(e, dt);
+    if (rc) return rc;
+    if (!e->ag_f32_valid && (rc = rebuild_f32_agents(e))) return rc;
+    return BFA_OK;
+}
+
+int bfa_diffusion_decay(bfa_env* e, double dt) {
+    if (!e) return BFA_ERR_ARG;
+    int rc = ready(e, dt);
+    if (rc) return rc;
+    return enqueue_diffusion(e);
+}
+
+int bfa_secretion_uptake(bfa_env* e) {
+    if (!e) return BFA_ERR_ARG;
+    int rc;
+    if (!e->ag_f32_valid && (rc = rebuild_f32_agents(e))) return rc;
+    return launch_agents(e);
+}
+
+int bfa_step(bfa_env* e, double dt, int nsteps) {
+    if (!e) return BFA_ERR_ARG;
+    if (nsteps < 0) return fail(e, BFA_ERR_ARG, "negative step count");
+    int rc = ready(e, dt);
+    if (rc) return rc;
+    if (nsteps == 0) return BFA_OK;
+    if (!e->graph_valid || e->graph_dt != dt) {
+        if ((rc = build_graph(e, dt))) return rc;
+    }
+    for (int s = 0; s < nsteps; ++s) HIP_TRY(e, hipGraphLaunch(e->gexec, e->stream));
+    return BFA_OK;
+}
+
+int bfa_apply_dirichlet(bfa_env* e) {
+    if (!e) return BFA_ERR_ARG;
+    // the Dirichlet table lives with the precompute; any dt works for it
+    const double dt = e->pre_valid ? e->pre_dt : 1.0;
+    int rc = ensure_precompute(e, dt);
+    if (rc) return rc;
+    const long long work = 2ll * e->ny * e->nz + 2ll * e->nx * e->nz + 2ll * e->nx * e->ny;
+    const unsigned blocks = (unsigned)((work + 255) / 256);
+    if (e->prec == BFA_FP32)
+        hipLaunchKernelGGL(bfa::k_dirichlet_faces<float>, dim3(blocks), dim3(256), 0, e->stream,
+                           sweep_args<float>(e, 0));
+    else
+        hipLaunchKernelGGL(bfa::k_dirichlet_faces<double>, dim3(blocks), dim3(256), 0, e->stream,
+                           sweep_args<double>(e, 0));
+    LAUNCH_CHECK(e, "dirichlet faces");
+    return BFA_OK;
+}
+
+int bfa_total_mass(bfa_env* e, double* out) {
+    if (!e || !out) return fail(e, BFA_ERR_ARG, "NULL argument");
+    std::vector<double> sum(e->ns);
+    int rc = reduce(e, e->u, e->prec == BFA_FP32, e->N, 1, e->N, 0, nullptr, sum.data());
+    if (rc) return rc;
+    const double dV = e->dx * e->dx * e->dx;
+    for (int d = 0; d < e->ns; ++d)
+        out[d] = (e->prec == BFA_FP32 ? (double)e->N * e->b[d] + e->s[d] * sum[d] : sum[d]) * dV;
+    return BFA_OK;
+}
+
+int bfa_synchronize(bfa_env* e) {
+    if (!e) return BFA_ERR_ARG;
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    return BFA_OK;
+}
+
+int bfa_time_kernels(bfa_env* e, double dt, int reps, double* ms) {
+    if (!e || !ms || reps < 1) return fail(e, BFA_ERR_ARG, "bad argument");
+    int rc = ready(e, dt);
+    if (rc) return rc;
+    hipEvent_t ev[6];
+    for (auto& x : ev) HIP_TRY(e, hipEventCreate(&x));
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int r = 0; r < reps && rc == BFA_OK; ++r) {
+        (void)hipEventRecord(ev[0], e->stream);
+        for (int a = 0; a < 3 && rc == BFA_OK; ++a) {
+            rc = launch_sweep_any(e, a);
+            (void)hipEventRecord(ev[a + 1], e->stream);
+        }
+        if (rc == BFA_OK) rc = launch_agents(e);
+        (void)hipEventRecord(ev[4], e->stream);
+        (void)hipEventSynchronize(ev[4]);
+        float t;
+        for (int k = 0; k < 4; ++k) {
+            (void)hipEventElapsedTime(&t, ev[k], ev[k + 1]);
+            acc[k] += t;
+        }
+        (void)hipEventElapsedTime(&t, ev[0], ev[4]);
+        acc[4] += t;
+    }
+    for (auto& x : ev) (void)hipEventDestroy(x);
+    if (rc) return rc;
+    for (int k = 0; k < 5; ++k) ms[k] = acc[k] / reps;
+    return BFA_OK;
+}
+
+int bfa_copy_kernel(size_t bytes, int reps, double* ms) {
+    if (!ms || reps < 1 || bytes < 16) return BFA_ERR_ARG;
+    const long long n = (long long)(bytes / 16);
+    uint4 *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, n * 16) != hipSuccess) return BFA_ERR_NOMEM;
+    if (hipMalloc(&b, n * 16) != hipSuccess) {
+        (void)hipFree(a);
+        return BFA_ERR_NOMEM;
+    }
+    (void)hipMemset(a, 1, n * 16);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const unsigned blocks = (unsigned)std::min<long long>(65536, (n + 255) / 256);
+    hipLaunchKernelGGL(bfa::k_copy16, dim3(blocks), dim3(256), 0, nullptr, a, b, n);
+    (void)hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(bfa::k_copy16, dim3(blocks), dim3(256), 0, nullptr, a, b, n);
+    (void)hipEventRecord(e1, nullptr);
+    hipError_t st = hipEventSynchronize(e1);
+    float t = 0;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (st != hipSuccess) return BFA_ERR_HIP;
+    *ms = t / reps;
+    return BFA_OK;
+}
+
+int bfa_set_tiles(bfa_env* e, int Lx, int Ly, int Lz, int rows_x) {
+    if (!e) return BFA_ERR_ARG;
+    const int save[4] = {e->Lreq[0], e->Lreq[1], e->Lreq[2], e->Rreq};
+    const int saveL[4] = {e->L[0], e->L[1], e->L[2], e->R};
+    e->Lreq[0] = Lx;
+    e->Lreq[1] = Ly;
+    e->Lreq[2] = Lz;
+    e->Rreq = rows_x;
+    int rc = apply_tiles(e);
+    if (rc) {
+        e->Lreq[0] = save[0];
+        e->Lreq[1] = save[1];
+        e->Lreq[2] = save[2];
+        e->Rreq = save[3];
+        e->L[0] = saveL[0];
+        e->L[1] = saveL[1];
+        e->L[2] = saveL[2];
+        e->R = saveL[3];
+        return rc;
+    }
+    invalidate_graph(e);
+    return BFA_OK;
+}
+
+int bfa_get_tiles(const bfa_env* e, int* Lx, int* Ly, int* Lz, int* rows_x) {
+    if (!e || !Lx || !Ly || !Lz || !rows_x) return BFA_ERR_ARG;
+    *Lx = e->L[0];
+    *Ly = e->L[1];
+    *Lz = e->L[2];
+    *rows_x = e->R;
+    return BFA_OK;
+}
+
+}  // extern "C"

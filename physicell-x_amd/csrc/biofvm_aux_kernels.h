@@ -1,0 +1,293 @@
+// biofvm_aux_kernels.h -- gfx950 device code around the sweeps: Thomas
+// precompute, agent secretion/uptake, Dirichlet faces, reductions and the
+// layout conversions at the host boundary.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "biofvm_kernels.h"
+
+namespace bfa {
+
+// ---- Thomas precompute (FP64), one thread per substrate --------------------
+// Constants as BioFVM/BioFVM_solvers.cpp:194-217, pivots and c' as :226-329:
+//   c1 = D*dt/dx/dx, c2 = lambda*dt/3, c3 = 1+c1+c1+c2, c3a = 1+c1+c2,
+//   den[0] = c3a (1+c2 when n == 1), cp[0] = -c1/den[0],
+//   den[i] = (i == n-1 ? c3a : c3) + c1*cp[i-1], cp[i] = -c1/den[i].
+// Writes den/cp in FP64 and 1/den, cp, c1 in the sweep type T.  For the FP32
+// deviation form (u = b + s*v) it also writes beta = (b/s)*c2: one sweep maps
+// v to T^-1 (v - beta) because every axis matrix has unit row sums plus c2.
+template <typename T>
+__global__ void k_thomas(int n, int ns, const double* __restrict__ params, double dt, double dx,
+                         double* __restrict__ den64, double* __restrict__ cp64,
+                         T* __restrict__ invT, T* __restrict__ cpT, T* __restrict__ kT,
+                         T* __restrict__ betaT) {
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= ns) return;
+    const double D = params[d], lam = params[ns + d], b = params[2 * ns + d],
+                 s = params[3 * ns + d];
+    double c1 = D;
+    c1 *= dt;
+    c1 /= dx;
+    c1 /= dx;
+    double c2 = lam;
+    c2 *= dt;
+    c2 /= 3.0;
+    double c3 = 1.0;
+    c3 += c1;
+    c3 += c1;
+    c3 += c2;
+    double c3a = 1.0;
+    c3a += c1;
+    c3a += c2;
+    double* den = den64 + (size_t)d * n;
+    double* cp = cp64 + (size_t)d * n;
+    den[0] = (n == 1) ? 1.0 + c2 : c3a;
+    cp[0] = (-c1) / den[0];
+    for (int i = 1; i < n; ++i) {
+        const double base = (i == n - 1) ? c3a : c3;
+        den[i] = fma(c1, cp[i - 1], base);
+        cp[i] = (-c1) / den[i];
+    }
+    for (int i = 0; i < n; ++i) {
+        invT[(size_t)d * n + i] = (T)(1.0 / den[i]);
+        cpT[(size_t)d * n + i] = (T)cp[i];
+    }
+    kT[d] = (T)c1;
+    betaT[d] = (T)((b / s) * c2);
+}
+
+// ---- agents: ordered per-voxel composition ----------------------------------
+// Agents are sorted by device voxel (stable in agent index) on the host, so
+// each segment [off[q], off[q+1]) holds the agents of one voxel in ascending
+// agent order.  One lane per (segment, substrate) walks its segment in order:
+// the reference's serial loop order (BioFVM/BioFVM_microenvironment.cpp:
+// 1370-1384) with the per-agent update of BioFVM/BioFVM_basic_agent.cpp:
+// 536-552.  FP64: rho = (rho + c1)/c2 + e2 in the reference's operation
+// order.  FP32 deviation form: v = A*v + B with A = 1/c2 and
+// B = ((b + c1)/c2 + e2 - b)/s precomputed in FP64.
+// Constants are stored [ns][nagents_sorted] so lanes of one substrate read
+// consecutive entries.
+__global__ void k_agents_f64(double* __restrict__ u, long long N, int ns, int nseg,
+                             int nag, const int* __restrict__ off,
+                             const long long* __restrict__ vox, const double* __restrict__ c1,
+                             const double* __restrict__ c2, const double* __restrict__ e2) {
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (long long)nseg * ns) return;
+    const int d = (int)(gid / nseg);
+    const int q = (int)(gid - (long long)d * nseg);
+    double* p = u + (long long)d * N + vox[q];
+    double rho = *p;
+    const int a0 = off[q], a1 = off[q + 1];
+    const size_t o = (size_t)d * nag;
+    for (int a = a0; a < a1; ++a) {
+        rho += c1[o + a];
+        rho /= c2[o + a];
+        rho += e2[o + a];
+    }
+    *p = rho;
+}
+
+__global__ void k_agents_f32(float* __restrict__ u, long long N, int ns, int nseg, int nag,
+                             const int* __restrict__ off, const long long* __restrict__ vox,
+                             const float* __restrict__ A, const float* __restrict__ B) {
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (long long)nseg * ns) return;
+    const int d = (int)(gid / nseg);
+    const int q = (int)(gid - (long long)d * nseg);
+    float* p = u + (long long)d * N + vox[q];
+    float v = *p;
+    const int a0 = off[q], a1 = off[q + 1];
+    const size_t o = (size_t)d * nag;
+    for (int a = a0; a < a1; ++a) v = fmaf(A[o + a], v, B[o + a]);
+    *p = v;
+}
+
+// ---- standalone Dirichlet pass over the six faces ---------------------------
+// Thread space: [x faces 2*ny*nz][y faces 2*nx*nz][z faces 2*nx*ny]; voxels
+// on edges are written more than once, always with the same value.
+template <typename T>
+__global__ void k_dirichlet_faces(SweepArgs<T> a) {
+    const long long nxf = 2ll * a.ny * a.nz, nyf = 2ll * a.nx * a.nz, nzf = 2ll * a.nx * a.ny;
+    long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (g < nxf) {
+        i = (g & 1) ? a.nx - 1 : 0;
+        g >>= 1;
+        k = (int)(g % a.nz);
+        j = (int)(g / a.nz);
+    } else if ((g -= nxf) < nyf) {
+        j = (g & 1) ? a.ny - 1 : 0;
+        g >>= 1;
+        k = (int)(g % a.nz);
+        i = (int)(g / a.nz);
+    } else if ((g -= nyf) < nzf) {
+        k = (g & 1) ? a.nz - 1 : 0;
+        g >>= 1;
+        j = (int)(g % a.ny);
+        i = (int)(g / a.ny);
+    } else {
+        return;
+    }
+    const int slot = dir_slot(i, j, k, a.nx, a.ny, a.nz, a.reg);
+    const long long v = ((long long)k * a.ny + j) * a.nx + i;
+    for (int d = 0; d < a.ns; ++d)
+        if ((a.active >> d) & 1ull) a.u[(long long)d * a.N + v] = a.dval[slot * a.ns + d];
+}
+
+// ---- reductions --------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+    return v;
+}
+
+// Partial sums of x[off_d + v*stride], v < count, for substrate d = blockIdx.y
+// (off_d = d*dstride).  MODE 0: sum of x; MODE 1: max |x - ref[d]|.  Each
+// block owns a fixed contiguous slice, so the result does not depend on
+// scheduling.  blockDim.x = 256.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_reduce_partial(const T* __restrict__ x, long long count,
+                                                         long long stride, long long dstride,
+                                                         const double* __restrict__ ref,
+                                                         double* __restrict__ partial) {
+    __shared__ double sh[4];
+    const int d = blockIdx.y;
+    const long long per = (count + gridDim.x - 1) / gridDim.x;
+    const long long v0 = (long long)blockIdx.x * per;
+    const long long v1 = (v0 + per < count) ? v0 + per : count;
+    const T* base = x + (long long)d * dstride;
+    const double r = MODE == 1 ? ref[d] : 0.0;
+    double acc = 0.0;
+    for (long long v = v0 + threadIdx.x; v < v1; v += blockDim.x) {
+        const double e = (double)base[v * stride];
+        if (MODE == 0)
+            acc += e;
+        else
+            acc = fmax(acc, fabs(e - r));
+    }
+    acc = MODE == 0 ? wave_sum(acc) : wave_max(acc);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tot = sh[0];
+        for (int w = 1; w < 4; ++w) tot = MODE == 0 ? tot + sh[w] : fmax(tot, sh[w]);
+        partial[(size_t)d * gridDim.x + blockIdx.x] = tot;
+    }
+}
+
+// Fixed-order final reduction of nb partials per substrate; one block (64
+// threads) per substrate.
+template <int MODE>
+__global__ void k_reduce_final(const double* __restrict__ partial, int nb,
+                               double* __restrict__ out) {
+    const int d = blockIdx.x;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 64) {
+        const double e = partial[(size_t)d * nb + b];
+        acc = MODE == 0 ? acc + e : fmax(acc, e);
+    }
+    acc = MODE == 0 ? wave_sum(acc) : wave_max(acc);
+    if (threadIdx.x == 0) out[d] = acc;
+}
+
+// ---- host-layout conversion ----------------------------------------------------
+// Reference layout in[((i*ny + j)*nz + k)*ns + d] (FP64) to device layout
+// out[((d*nz + k)*ny + j)*nx + i] (T), through a 32x32 LDS tile per (j,
+// i-block, q-block) with q = k*ns + d, so reads are contiguous in q and
+// writes contiguous in i.  v = (u - b[d]) * rs[d] with rs = 1/s (s a power
+// of two); FP64 passes b = 0, rs = 1.  blockDim = (32, 8); grid =
+// (ceil(nz*ns/32), ceil(nx/32), ny).
+template <typename T>
+__global__ __launch_bounds__(256) void k_ref_to_soa(const double* __restrict__ in,
+                                                     T* __restrict__ out, int nx, int ny,
+                                                     int nz, int ns,
+                                                     const double* __restrict__ b,
+                                                     const double* __restrict__ rs) {
+    __shared__ double tile[32][33];
+    const long long Q = (long long)nz * ns;
+    const long long q0 = (long long)blockIdx.x * 32;
+    const int i0 = blockIdx.y * 32;
+    const int j = blockIdx.z;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + r;
+        const long long q = q0 + tx;
+        if (i < nx && q < Q) tile[r][tx] = in[((long long)i * ny + j) * Q + q];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const long long q = q0 + r;
+        const int i = i0 + tx;
+        if (i < nx && q < Q) {
+            const int k = (int)(q / ns), d = (int)(q - (long long)k * ns);
+            const double v = (tile[tx][r] - b[d]) * rs[d];
+            out[(((long long)d * nz + k) * ny + j) * nx + i] = (T)v;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_soa_to_ref(const T* __restrict__ in,
+                                                     double* __restrict__ out, int nx, int ny,
+                                                     int nz, int ns,
+                                                     const double* __restrict__ b,
+                                                     const double* __restrict__ s) {
+    __shared__ double tile[32][33];
+    const long long Q = (long long)nz * ns;
+    const long long q0 = (long long)blockIdx.x * 32;
+    const int i0 = blockIdx.y * 32;
+    const int j = blockIdx.z;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    for (int r = ty; r < 32; r += 8) {
+        const long long q = q0 + r;
+        const int i = i0 + tx;
+        if (i < nx && q < Q) {
+            const int k = (int)(q / ns), d = (int)(q - (long long)k * ns);
+            tile[r][tx] = fma((double)in[(((long long)d * nz + k) * ny + j) * nx + i], s[d], b[d]);
+        }
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + r;
+        const long long q = q0 + tx;
+        if (i < nx && q < Q) out[((long long)i * ny + j) * Q + q] = tile[tx][r];
+    }
+}
+
+// Device-layout FP64 to T and back, elementwise (substrate d = blockIdx.y).
+template <typename T>
+__global__ void k_soa_from_f64(const double* __restrict__ in, T* __restrict__ out, long long N,
+                               const double* __restrict__ b, const double* __restrict__ rs) {
+    const int d = blockIdx.y;
+    const double bb = b[d], r = rs[d];
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < N;
+         v += (long long)gridDim.x * blockDim.x)
+        out[(long long)d * N + v] = (T)((in[(long long)d * N + v] - bb) * r);
+}
+
+template <typename T>
+__global__ void k_soa_to_f64(const T* __restrict__ in, double* __restrict__ out, long long N,
+                             const double* __restrict__ b, const double* __restrict__ s) {
+    const int d = blockIdx.y;
+    const double bb = b[d], ss = s[d];
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < N;
+         v += (long long)gridDim.x * blockDim.x)
+        out[(long long)d * N + v] = fma((double)in[(long long)d * N + v], ss, bb);
+}
+
+// Plain 16-byte copy, used only to calibrate HBM byte counters.
+__global__ void k_copy16(const uint4* __restrict__ in, uint4* __restrict__ out, long long n) {
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (long long)gridDim.x * blockDim.x)
+        out[v] = in[v];
+}
+
+}  // namespace bfa

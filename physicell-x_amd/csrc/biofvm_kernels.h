@@ -1,0 +1,315 @@
+// biofvm_kernels.h -- gfx950 device code for the BioFVM LOD diffusion-decay step.
+//
+// Device layout (one microenvironment): the field is substrate-major SoA,
+//   u[((d*nz + k)*ny + j)*nx + i],
+// so x is the unit-stride axis.  One LOD step is three sweeps, each solving
+// the constant-coefficient tridiagonal system of every grid line along one
+// axis (BioFVM/BioFVM_solvers.cpp:352-777), with the reference's four
+// Dirichlet passes (:345,682,731,778) fused into the sweeps: the first and
+// second into the x sweep's prologue and epilogue, the third into the y
+// sweep's epilogue, the fourth into the z sweep's epilogue.
+//
+// Line solve.  For a line r[0..n-1] the reference computes
+//   forward   y[0] = r[0]/den[0],  y[i] = (r[i] + k*y[i-1])/den[i]
+//   backward  x[n-1] = y[n-1],     x[i] = y[i] - cp[i]*x[i+1]
+// with den/cp the Thomas LU pivots and c' coefficients.  Here a line is cut
+// into P chunks of L consecutive nodes, one chunk per thread, held in
+// registers.  Both recurrences are first-order linear, so a chunk's result is
+// its zero-carry result plus a carry-in times the product of its
+// multipliers.  Each thread computes (zero-carry end value, multiplier
+// product) for its chunk, the P pairs are exchanged through LDS, every thread
+// folds the pairs of the chunks before it (forward) or after it (backward) to
+// obtain its exact carry-in, then reruns the chunk recurrence from that
+// carry.  Every field value is read from HBM once and written once per sweep.
+//
+// Loads are coalesced on every axis: the y and z sweeps give each lane one
+// x-column (64 adjacent columns per wavefront), and the x sweep stages a tile
+// of R whole rows through LDS with an odd row pitch, so lanes that own the
+// same chunk of different rows hit distinct banks.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bfa {
+
+// Face selection for a boundary voxel: the last registered face (order xmin,
+// xmax, ymin, ymax, zmin, zmax) containing it, else slot 6 (value 1.0).
+// Mirrors the add_dirichlet_node replay of
+// BioFVM/BioFVM_microenvironment.cpp:218-226,2649-2786.
+__device__ __forceinline__ int dir_slot(int i, int j, int k, int nx, int ny, int nz,
+                                        unsigned reg) {
+    if (k == nz - 1 && (reg & 32u)) return 5;
+    if (k == 0 && (reg & 16u)) return 4;
+    if (j == ny - 1 && (reg & 8u)) return 3;
+    if (j == 0 && (reg & 4u)) return 2;
+    if (i == nx - 1 && (reg & 2u)) return 1;
+    if (i == 0 && (reg & 1u)) return 0;
+    return 6;
+}
+
+template <typename T>
+struct SweepArgs {
+    T* u;             // field base of this microenvironment (SoA)
+    const T* inv;     // [ns][n] 1/den for the swept axis
+    const T* cp;      // [ns][n] c' for the swept axis
+    const T* k;       // [ns] D*dt/dx^2
+    const T* beta;    // [ns] deviation shift (FP32 form), unused for FP64
+    const T* dval;    // [7][ns] Dirichlet value per face slot
+    unsigned long long active;  // bit d set: substrate d has Dirichlet on
+    long long N;      // nx*ny*nz
+    int nx, ny, nz, ns;
+    unsigned reg;     // bit f set: face f registered
+};
+
+// ---- chunk recurrences (positions s..s+L-1 of a line of length n) ----------
+
+template <typename T, int L>
+__device__ __forceinline__ void fwd_local(const T (&x)[L], int s, int n,
+                                          const T* __restrict__ inv, T k, T& zend, T& G) {
+    T z = T(0), g = T(1);
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+        if (s + m < n) {
+            const T iv = inv[s + m];
+            z = fma(k, z, x[m]) * iv;
+            g = g * (k * iv);
+        }
+    }
+    zend = z;
+    G = g;
+}
+
+template <typename T, int L>
+__device__ __forceinline__ void fwd_apply(T (&x)[L], int s, int n, const T* __restrict__ inv,
+                                          T k, T y) {
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+        if (s + m < n) {
+            y = fma(k, y, x[m]) * inv[s + m];
+            x[m] = y;
+        }
+    }
+}
+
+template <typename T, int L>
+__device__ __forceinline__ void bwd_local(const T (&x)[L], int s, int n,
+                                          const T* __restrict__ cp, T& wstart, T& H) {
+    T w = T(0), h = T(1);
+#pragma unroll
+    for (int m = L - 1; m >= 0; --m) {
+        if (s + m < n) {
+            const T c = cp[s + m];
+            w = fma(-c, w, x[m]);
+            h = h * (-c);
+        }
+    }
+    wstart = w;
+    H = h;
+}
+
+template <typename T, int L>
+__device__ __forceinline__ void bwd_apply(T (&x)[L], int s, int n, const T* __restrict__ cp,
+                                          T xn) {
+#pragma unroll
+    for (int m = L - 1; m >= 0; --m) {
+        if (s + m < n) {
+            xn = fma(-cp[s + m], xn, x[m]);
+            x[m] = xn;
+        }
+    }
+}
+
+// Full chunked solve of one line chunk held in x[].  s_z holds one carry per
+// (chunk, line) at s_z[p*lines + line]; s_g one multiplier product per chunk.
+// All threads of the block must call this (it contains barriers); threads
+// with act == false only take part in the barriers.
+template <typename T, int L>
+__device__ __forceinline__ void chunk_solve(T (&x)[L], bool act, int p, int P, int line,
+                                            int lines, int n, const T* __restrict__ s_inv,
+                                            const T* __restrict__ s_cp, T k, T* s_z, T* s_g) {
+    const int s = p * L;
+    T a = T(0), g = T(1);
+    if (act) {
+        fwd_local<T, L>(x, s, n, s_inv, k, a, g);
+        s_z[p * lines + line] = a;
+        if (line == 0) s_g[p] = g;
+    }
+    __syncthreads();
+    if (act) {
+        T y = T(0);
+        for (int q = 0; q < p; ++q) y = fma(s_g[q], y, s_z[q * lines + line]);
+        fwd_apply<T, L>(x, s, n, s_inv, k, y);
+    }
+    __syncthreads();
+    if (act) {
+        bwd_local<T, L>(x, s, n, s_cp, a, g);
+        s_z[p * lines + line] = a;
+        if (line == 0) s_g[p] = g;
+    }
+    __syncthreads();
+    if (act) {
+        T xn = T(0);
+        for (int q = P - 1; q > p; --q) xn = fma(s_g[q], xn, s_z[q * lines + line]);
+        bwd_apply<T, L>(x, s, n, s_cp, xn);
+    }
+}
+
+// ---- x sweep: R rows per block staged through LDS ---------------------------
+// blockDim.x = roundup(R*P, 64) with P = ceil(nx/L); grid = (ceil(ny*nz/R), ns).
+// Dynamic LDS: (R*S + 2*nx + P*R + P) * sizeof(T), S = nx | 1.
+template <typename T, int L, bool DEV>
+__global__ __launch_bounds__(1024) void k_sweep_x(SweepArgs<T> a, int R, int logR) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int P = (nx + L - 1) / L;
+    const int S = nx | 1;
+    T* tile = reinterpret_cast<T*>(smem);
+    T* s_inv = tile + (size_t)R * S;
+    T* s_cp = s_inv + nx;
+    T* s_z = s_cp + nx;
+    T* s_g = s_z + (size_t)P * R;
+
+    const int t = threadIdx.x;
+    const int nthr = blockDim.x;
+    const int d = blockIdx.y;
+    const long long rows = (long long)ny * nz;
+    const long long r0 = (long long)blockIdx.x * R;
+    const int rvalid = (int)((rows - r0) < R ? (rows - r0) : R);
+    T* g = a.u + (long long)d * a.N + r0 * nx;
+
+    for (int i = t; i < nx; i += nthr) {
+        s_inv[i] = a.inv[(long long)d * nx + i];
+        s_cp[i] = a.cp[(long long)d * nx + i];
+    }
+    const int wave = t >> 6, lane = t & 63, nw = nthr >> 6;
+    for (int row = wave; row < rvalid; row += nw) {
+        const T* src = g + (long long)row * nx;
+        T* dst = tile + row * S;
+        for (int col = lane; col < nx; col += 64) dst[col] = src[col];
+    }
+    __syncthreads();
+
+    const int rho = t & (R - 1);
+    const int p = t >> logR;
+    const bool act = (p < P) && (rho < rvalid);
+    const int s = p * L;
+    int jj = 0, kk = 0;
+    bool rowb = false;
+    if (act) {
+        const long long r = r0 + rho;
+        kk = (int)(r / ny);
+        jj = (int)(r - (long long)kk * ny);
+        rowb = (jj == 0 || jj == ny - 1 || kk == 0 || kk == nz - 1);
+    }
+    const bool dact = (a.active >> d) & 1ull;
+    T x[L];
+#pragma unroll
+    for (int m = 0; m < L; ++m) x[m] = (act && s + m < nx) ? tile[rho * S + s + m] : T(0);
+    if (act && dact) {
+#pragma unroll
+        for (int m = 0; m < L; ++m) {
+            const int pos = s + m;
+            if (pos < nx && (rowb || pos == 0 || pos == nx - 1))
+                x[m] = a.dval[dir_slot(pos, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
+        }
+    }
+    if (DEV) {
+        const T bt = a.beta[d];
+#pragma unroll
+        for (int m = 0; m < L; ++m) x[m] -= bt;
+    }
+    chunk_solve<T, L>(x, act, p, P, rho, R, nx, s_inv, s_cp, a.k[d], s_z, s_g);
+    if (act && dact) {
+#pragma unroll
+        for (int m = 0; m < L; ++m) {
+            const int pos = s + m;
+            if (pos < nx && (rowb || pos == 0 || pos == nx - 1))
+                x[m] = a.dval[dir_slot(pos, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
+        }
+    }
+    if (act) {
+#pragma unroll
+        for (int m = 0; m < L; ++m)
+            if (s + m < nx) tile[rho * S + s + m] = x[m];
+    }
+    __syncthreads();
+    for (int row = wave; row < rvalid; row += nw) {
+        T* dst = g + (long long)row * nx;
+        const T* src = tile + row * S;
+        for (int col = lane; col < nx; col += 64) dst[col] = src[col];
+    }
+}
+
+// ---- y / z sweeps: one lane per x-column, 64 columns per block --------------
+// AXIS 1 = y (line stride nx, columns (k,i)), AXIS 2 = z (stride nx*ny,
+// columns (j,i)).  blockDim.x = 64*P with P = ceil(n/L);
+// grid = (ceil(ncols/64), ns).  Dynamic LDS: (2*n + 64*P + P) * sizeof(T).
+template <typename T, int L, int AXIS, bool DEV>
+__global__ __launch_bounds__(1024) void k_sweep_col(SweepArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int n = AXIS == 1 ? ny : nz;
+    const int P = (n + L - 1) / L;
+    T* s_inv = reinterpret_cast<T*>(smem);
+    T* s_cp = s_inv + n;
+    T* s_z = s_cp + n;
+    T* s_g = s_z + (size_t)64 * P;
+
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int p = t >> 6;
+    const int d = blockIdx.y;
+    for (int i = t; i < n; i += blockDim.x) {
+        s_inv[i] = a.inv[(long long)d * n + i];
+        s_cp[i] = a.cp[(long long)d * n + i];
+    }
+    const long long ncols = AXIS == 1 ? (long long)nz * nx : (long long)ny * nx;
+    const long long c = (long long)blockIdx.x * 64 + lane;
+    const bool act = c < ncols;
+    long long base = 0, stride;
+    int ci = 0, co = 0;  // i and the other transverse coordinate (k for y, j for z)
+    if (act) {
+        co = (int)(c / nx);
+        ci = (int)(c - (long long)co * nx);
+    }
+    if (AXIS == 1) {
+        base = (long long)d * a.N + (long long)co * ny * nx + ci;
+        stride = nx;
+    } else {
+        base = (long long)d * a.N + c;
+        stride = (long long)nx * ny;
+    }
+    const int s = p * L;
+    T x[L];
+#pragma unroll
+    for (int m = 0; m < L; ++m) x[m] = (act && s + m < n) ? a.u[base + (s + m) * stride] : T(0);
+    if (DEV) {
+        const T bt = a.beta[d];
+#pragma unroll
+        for (int m = 0; m < L; ++m) x[m] -= bt;
+    }
+    __syncthreads();  // coefficient staging complete
+    chunk_solve<T, L>(x, act, p, P, lane, 64, n, s_inv, s_cp, a.k[d], s_z, s_g);
+    const bool dact = (a.active >> d) & 1ull;
+    if (act && dact) {
+        const bool lineb = AXIS == 1 ? (ci == 0 || ci == nx - 1 || co == 0 || co == nz - 1)
+                                     : (ci == 0 || ci == nx - 1 || co == 0 || co == ny - 1);
+#pragma unroll
+        for (int m = 0; m < L; ++m) {
+            const int pos = s + m;
+            if (pos < n && (lineb || pos == 0 || pos == n - 1)) {
+                const int slot = AXIS == 1 ? dir_slot(ci, pos, co, nx, ny, nz, a.reg)
+                                           : dir_slot(ci, co, pos, nx, ny, nz, a.reg);
+                x[m] = a.dval[slot * a.ns + d];
+            }
+        }
+    }
+    if (act) {
+#pragma unroll
+        for (int m = 0; m < L; ++m)
+            if (s + m < n) a.u[base + (s + m) * stride] = x[m];
+    }
+}
+
+}  // namespace bfa

@@ -1,0 +1,283 @@
+"""GPU parity: the HIP library against the CPU oracle on identical seeded inputs.
+
+Bars (relative max-norm, max|gpu - oracle| / max|oracle|):
+  * FP64 path: <= 1e-12 after every step;
+  * FP32 scaled-deviation path: <= 1e-5 after every step, per substrate;
+  * layout round trips and the Thomas coefficients: exact or <= 4 ulp.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import agents_for, gpu_env, load_pkg, make_problem, oracle_for, relmax
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def pkg():
+    return load_pkg()
+
+
+def run_compare(pkg, p, steps, precision, tol, per_substrate=False, use_step=False):
+    orc = oracle_for(p)
+    ag = agents_for(p) if p.voxel.size else None
+    u = p.u0.copy()
+    env = gpu_env(pkg, p, precision)
+    worst = 0.0
+    try:
+        for s in range(steps):
+            orc.diffusion_step(u, p.dt)
+            if ag is not None:
+                orc.secretion_uptake(u, ag)
+            if use_step:
+                env.step(p.dt, 1)
+            else:
+                env.simulate_diffusion_decay(p.dt)
+                env.simulate_cell_sources_and_sinks(p.dt)
+            g = env.download()
+            if per_substrate:
+                gg = g.reshape(-1, p.ns)
+                uu = u.reshape(-1, p.ns)
+                err = max(relmax(gg[:, d], uu[:, d]) for d in range(p.ns))
+            else:
+                err = relmax(g, u)
+            worst = max(worst, err)
+            assert err <= tol, f"step {s}: relative max-norm error {err:.3e} > {tol:.0e}"
+    finally:
+        env.close()
+    return worst
+
+
+@pytest.mark.parametrize("shape", [(17, 23, 29, 1), (32, 32, 32, 2), (33, 7, 65, 3),
+                                   (1, 5, 9, 2), (64, 1, 3, 1), (5, 130, 2, 4)])
+def test_fp64_diffusion_matches_oracle(pkg, shape):
+    p = make_problem(*shape, seed=sum(shape), n_agents=0, dirichlet="all")
+    run_compare(pkg, p, 3, "fp64", FP64_TOL)
+
+
+@pytest.mark.parametrize("dirichlet", ["all", "mixed", "off"])
+def test_fp64_dirichlet_variants(pkg, dirichlet):
+    p = make_problem(24, 19, 21, 3, seed=5, n_agents=50, dirichlet=dirichlet, export=20.0)
+    run_compare(pkg, p, 4, "fp64", FP64_TOL)
+
+
+def test_fp64_config1_workload(pkg):
+    """100^3 voxels, one substrate (D=1e5, decay 0.1), Dirichlet 38 on all
+    faces, uniform 38 start, 1000 uptake-only agents (U=10, V=2494), 100 steps."""
+    nx = ny = nz = 100
+    p = make_problem(nx, ny, nz, 1, seed=1, n_agents=0, dirichlet="all", noise=0.0)
+    p.D = np.array([1e5])
+    p.decay = np.array([0.1])
+    p.fv = np.full((6, 1), 38.0)
+    p.u0 = np.full(nx * ny * nz, 38.0)
+    rng = np.random.default_rng(1000)
+    bbox = [-1000.0, -1000.0, -1000.0, 1000.0, 1000.0, 1000.0]
+    pos = rng.uniform(-1000.0, 1000.0, (1000, 3))
+    p.voxel = oracle.voxel_indices(pos, bbox, 20.0, 20.0, 20.0, nx, ny, nz)
+    n = p.voxel.size
+    p.c1, p.c2, p.e1, p.e2 = oracle.agent_constants(0.01, 8000.0, np.full(n, 2494.0),
+                                                    np.zeros((n, 1)), np.full((n, 1), 10.0),
+                                                    np.zeros((n, 1)), np.zeros((n, 1)))
+    p.agent_active = np.ones(n, np.uint8)
+    orc = oracle_for(p)
+    ag = agents_for(p)
+    u = p.u0.copy()
+    env = gpu_env(pkg, p, "fp64")
+    try:
+        for s in range(100):
+            orc.diffusion_step(u, p.dt)
+            orc.secretion_uptake(u, ag)
+            env.step(p.dt, 1)
+            if s % 10 == 9 or s < 3:
+                g = env.download()
+                assert relmax(g, u) <= FP64_TOL, f"step {s}"
+                gm = env.total_mass()
+                om = orc.total_mass(u)
+                assert abs(gm[0] - om[0]) / abs(om[0]) <= 1e-12
+    finally:
+        env.close()
+
+
+def test_fp64_agents_sharing_voxels_follow_index_order(pkg):
+    p = make_problem(20, 18, 16, 2, seed=77, n_agents=400, dirichlet="mixed", export=30.0,
+                     collide=60)
+    run_compare(pkg, p, 3, "fp64", FP64_TOL)
+
+
+def test_fp64_agents_on_boundary_voxels(pkg):
+    p = make_problem(12, 10, 9, 2, seed=78, n_agents=0, dirichlet="all")
+    nx, ny, nz = p.nx, p.ny, p.nz
+    vox = [(0 * ny + 3) * nz + 4, ((nx - 1) * ny + 0) * nz + 0, (5 * ny + 5) * nz + (nz - 1),
+           (5 * ny + 5) * nz + 4, (5 * ny + 5) * nz + 4]
+    p.voxel = np.array(vox, np.int32)
+    rng = np.random.default_rng(3)
+    n = len(vox)
+    p.c1, p.c2, p.e1, p.e2 = oracle.agent_constants(0.01, 8000.0, np.full(n, 3000.0),
+                                                    rng.uniform(0, 10, (n, 2)), rng.uniform(0, 10, (n, 2)),
+                                                    rng.uniform(0, 50, (n, 2)), np.full((n, 2), 40.0))
+    p.agent_active = np.ones(n, np.uint8)
+    run_compare(pkg, p, 3, "fp64", FP64_TOL)
+
+
+def test_graph_step_equals_eager_calls(pkg):
+    p = make_problem(40, 36, 30, 2, seed=9, n_agents=300, dirichlet="mixed", export=5.0)
+    a = gpu_env(pkg, p, "fp64")
+    b = gpu_env(pkg, p, "fp64")
+    try:
+        a.step(p.dt, 5)
+        for _ in range(5):
+            b.simulate_diffusion_decay(p.dt)
+            b.simulate_cell_sources_and_sinks()
+        assert np.array_equal(a.download(), b.download())
+    finally:
+        a.close()
+        b.close()
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32, 2), (48, 40, 56, 4), (100, 100, 100, 1)])
+def test_fp32_deviation_form_matches_oracle(pkg, shape):
+    p = make_problem(*shape, seed=11 + shape[0], n_agents=200, dirichlet="all")
+    run_compare(pkg, p, 10, "fp32", FP32_TOL, per_substrate=True, use_step=True)
+
+
+def test_fp32_mixed_dirichlet_and_export(pkg):
+    p = make_problem(30, 26, 22, 3, seed=21, n_agents=300, dirichlet="mixed", export=10.0,
+                     collide=40)
+    run_compare(pkg, p, 8, "fp32", FP32_TOL, per_substrate=True)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_layout_round_trips(pkg, precision):
+    p = make_problem(19, 22, 25, 3, seed=4, dirichlet="off")
+    env = pkg.Microenvironment(p.nx, p.ny, p.nz, p.ns, p.dx, precision)
+    try:
+        env.set_coefficients(p.D, p.decay)
+        env.upload(p.u0, "ref")
+        soa = env.download("soa")
+        ref = env.download("ref")
+        exp_soa = p.u0.reshape(p.nx, p.ny, p.nz, p.ns).transpose(3, 2, 1, 0).ravel()
+        if precision == "fp64":
+            assert np.array_equal(ref, p.u0)
+            assert np.array_equal(soa, exp_soa)
+        else:
+            assert relmax(ref, p.u0) < 1e-7
+            assert relmax(soa, exp_soa) < 1e-7
+        env.upload(exp_soa, "soa")
+        back = env.download("ref")
+        assert relmax(back, p.u0) < (1e-7 if precision == "fp32" else 1e-300)
+    finally:
+        env.close()
+
+
+def test_total_mass_and_dirichlet_pass(pkg):
+    p = make_problem(21, 17, 13, 2, seed=8, dirichlet="mixed")
+    orc = oracle_for(p)
+    env = gpu_env(pkg, p, "fp64")
+    try:
+        u = p.u0.copy()
+        assert np.allclose(env.total_mass(), orc.total_mass(u), rtol=1e-13, atol=0)
+        orc.apply_dirichlet(u)
+        env.apply_dirichlet_boundaries_conditions()
+        assert np.array_equal(env.download(), u)
+    finally:
+        env.close()
+
+
+def test_golden_fixture_cases_on_gpu(pkg):
+    doc = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "lod_exact.json")))
+    unhex = lambda x: [unhex(v) for v in x] if isinstance(x, list) else (
+        float.fromhex(x) if isinstance(x, str) else x)
+    for c in doc["cases"]:
+        nx, ny, nz, ns = c["nx"], c["ny"], c["nz"], c["ns"]
+        dt, dx = unhex(c["dt"]), unhex(c["dx"])
+        env = pkg.Microenvironment(nx, ny, nz, ns, dx, "fp64")
+        try:
+            env.set_coefficients(unhex(c["D"]), unhex(c["decay"]))
+            env.set_dirichlet(c["dirichlet_active"], c["face_registered"], np.array(unhex(c["face_values"])))
+            ag = c["agents"]
+            if ag["voxel"]:
+                c1, c2, e1, e2 = oracle.agent_constants(dt, dx ** 3, unhex(ag["V"]), unhex(ag["S"]),
+                                                        unhex(ag["U"]), unhex(ag["E"]),
+                                                        unhex(ag["rho_star"]))
+                env.set_agents(ag["voxel"], c1, c2, e2, ag["active"])
+            for step in c["steps"]:
+                env.upload(np.array(unhex(step["input"])))
+                env.step(dt, 1)
+                got = env.download()
+                assert relmax(got, np.array(unhex(step["expected"]))) < 1e-12, c["name"]
+        finally:
+            env.close()
+
+
+def test_changing_dt_recomputes_coefficients(pkg):
+    p = make_problem(16, 16, 16, 1, seed=2, dirichlet="all")
+    orc = oracle_for(p)
+    u = p.u0.copy()
+    env = gpu_env(pkg, p, "fp64")
+    try:
+        for dt in (0.01, 0.02, 0.01):
+            orc.precompute(dt)
+            orc.diffusion_step(u, dt)
+            env.simulate_diffusion_decay(dt)
+        assert relmax(env.download(), u) <= FP64_TOL
+    finally:
+        env.close()
+
+
+@pytest.mark.parametrize("tiles", [(4, 4, 4, 1), (8, 16, 32, 8), (32, 8, 16, 32), (16, 32, 4, 64)])
+def test_tile_shapes_give_the_same_answer(pkg, tiles):
+    p = make_problem(40, 33, 47, 2, seed=31, n_agents=100, dirichlet="mixed")
+    base = gpu_env(pkg, p, "fp64")
+    other = gpu_env(pkg, p, "fp64")
+    try:
+        other.set_tiles(*tiles)
+        base.step(p.dt, 2)
+        other.step(p.dt, 2)
+        assert relmax(other.download(), base.download()) <= 1e-14
+    finally:
+        base.close()
+        other.close()
+
+
+def test_full_size_fp64_eigenmode_and_mass(pkg):
+    """400^3 x 4 at its real shape: with Dirichlet off and no agents a separable
+    Neumann cosine mode decays by prod 1/mu per step, and with zero decay the
+    total mass is conserved."""
+    nx = ny = nz = 400
+    ns = 4
+    D = np.array([1e3, 1e4, 5e4, 1e5])
+    decay = np.array([0.0, 0.0, 0.3, 1.0])
+    dt, dx = 0.01, 20.0
+    m = (1, 2, 3)
+    ci = np.cos(np.pi * m[0] * (np.arange(nx) + 0.5) / nx)
+    cj = np.cos(np.pi * m[1] * (np.arange(ny) + 0.5) / ny)
+    ck = np.cos(np.pi * m[2] * (np.arange(nz) + 0.5) / nz)
+    mode = (ck[:, None, None] * cj[None, :, None] * ci[None, None, :]).ravel()  # SoA order per substrate
+    amp = np.array([1.0, 2.0, -3.0, 0.5])
+    field = np.concatenate([20.0 + amp[d] * mode for d in range(ns)])
+    env = pkg.Microenvironment(nx, ny, nz, ns, dx, "fp64")
+    try:
+        env.set_coefficients(D, decay)
+        env.upload(field, "soa")
+        m0 = env.total_mass()
+        steps = 3
+        env.step(dt, steps)
+        got = env.download("soa").reshape(ns, -1)
+        m1 = env.total_mass()
+        for d in range(ns):
+            k = D[d] * dt / dx ** 2
+            lp = decay[d] * dt / 3
+            mu = np.prod([1 + lp + 2 * k * (1 - np.cos(np.pi * mm / n)) for mm, n in zip(m, (nx, ny, nz))])
+            const = 20.0 / (1 + lp) ** (3 * steps)
+            exp = const + amp[d] * mode / mu ** steps
+            assert relmax(got[d], exp) < 1e-12, d
+        assert abs(m1[0] - m0[0]) / m0[0] < 1e-12 and abs(m1[1] - m0[1]) / m0[1] < 1e-12
+    finally:
+        env.close()
