@@ -81,8 +81,10 @@ def make_workload(n, ns, n_agents, seed):
                 S=S, U=U, E=E, rs=rs, V=V)
 
 
-def make_env(pkg, w, n, ns, precision):
+def make_env(pkg, w, n, ns, precision, tiles=None):
     env = pkg.Microenvironment(n, n, n, ns, 20.0, precision)
+    if tiles is not None:
+        env.set_tiles(tiles["Lx"], tiles["Ly"], tiles["Lz"], tiles["rows_x"], tiles["cols_yz"])
     env.set_coefficients(w["D"], w["decay"])
     env.set_dirichlet(np.ones(ns, np.uint8), np.ones(6, np.uint8), np.tile(w["dval"], (6, 1)))
     env.upload(w["field"], "soa")
@@ -117,7 +119,7 @@ def load_traffic(kernel_key):
         return None
 
 
-def cpu_baseline(w, n, ns, budget_s=15.0, max_steps=6):
+def cpu_baseline(w, n, ns, budget_s=12.0, max_steps=40):
     sys.path.insert(0, ROOT)
     import oracle
 
@@ -156,6 +158,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -167,12 +171,24 @@ def main():
     w = make_workload(n, ns, args.agents, args.seed)
     updates = n ** 3 * ns
 
-    env32 = make_env(pkg, w, n, ns, "fp32")
+    tiles = {"fp32": None, "fp64": None}
+    if not args.no_tune:
+        import importlib
+        at = importlib.import_module("physicellx_amd.autotune")
+        for prec in ("fp32",) if args.no_fp64 else ("fp32", "fp64"):
+            r = at.autotune(pkg, n, n, n, ns, prec, dt)
+            tiles[prec] = r
+            if args.verbose:
+                for t in r["trials"]:
+                    print(prec, json.dumps(t), file=sys.stderr)
+    copy_gbs = 2 * 2 ** 30 / (pkg.copy_kernel_ms(2 ** 30, 10) * 1e-3) / 1e9
+
+    env32 = make_env(pkg, w, n, ns, "fp32", tiles["fp32"])
     wall32, ms32, gwall32 = time_gpu(env32, dt, args.steps, args.warmup)
     tiles32 = env32.tiles()
     res = {}
     if not args.no_fp64:
-        env64 = make_env(pkg, w, n, ns, "fp64")
+        env64 = make_env(pkg, w, n, ns, "fp64", tiles["fp64"])
         wall64, ms64, gwall64 = time_gpu(env64, dt, args.steps, args.warmup)
         # both have now run warmup + 2*steps steps from the same start
         f32 = env32.download("soa").reshape(ns, -1)
@@ -214,9 +230,11 @@ def main():
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": traffic},
         "kernel_ms": dict(x=ms32[0], y=ms32[1], z=ms32[2], agents=ms32[3]),
-        "step_alg_frac_of_peak": (updates * 6 * 4) / (1e-3 * wall32 / args.steps) / 1e9 / PEAK_HBM_GBS,
+        "step_alg_frac_of_peak": (updates * 6 * 4) / (wall32 / args.steps) / 1e9 / PEAK_HBM_GBS,
+        "copy_kernel_GBps": copy_gbs,
         "graph_ms_per_step": 1e3 * gwall32 / args.steps,
-        "tiles": dict(Lx=tiles32[0], Ly=tiles32[1], Lz=tiles32[2], rows_x=tiles32[3]),
+        "tiles": dict(Lx=tiles32[0], Ly=tiles32[1], Lz=tiles32[2], rows_x=tiles32[3],
+                      cols_yz=tiles32[4], autotuned=not args.no_tune),
     }
     out.update(res)
     if not args.no_cpu:

@@ -133,11 +133,16 @@ int bfa_synchronize(bfa_env *env);
 int bfa_time_kernels(bfa_env *env, double dt, int reps, double *ms);
 int bfa_copy_kernel(size_t bytes, int reps, double *ms);
 
-/* Kernel shape selection (the chunk length L of the in-register line solve
- * per axis, 0 = automatic; the x sweep's rows per LDS tile, 0 = automatic).
- * Used by the autotuner. */
-int bfa_set_tiles(bfa_env *env, int Lx, int Ly, int Lz, int rows_x);
-int bfa_get_tiles(const bfa_env *env, int *Lx, int *Ly, int *Lz, int *rows_x);
+/* Kernel shape selection, used by the autotuner: the chunk length L of the
+ * in-register line solve per axis (4, 8, 16, 32 or 64; 0 = automatic); the
+ * x-sweep form: rows_x == 0 selects the wave-local x sweep (a row's chunks
+ * live in one wavefront), rows_x in {1,2,4,...,64} the LDS-tiled x sweep
+ * with that many rows per tile; cols_yz, the x-columns per y/z sweep block
+ * (16, 32 or 64; 0 = automatic).  On error the previous shape is kept.
+ * bfa_get_tiles writes {Lx, Ly, Lz, rows_x, cols_yz} (cols_yz = -1 when the
+ * y and z sweeps use different widths). */
+int bfa_set_tiles(bfa_env *env, int Lx, int Ly, int Lz, int rows_x, int cols_yz);
+int bfa_get_tiles(const bfa_env *env, int *out5);
 
 #ifdef __cplusplus
 }

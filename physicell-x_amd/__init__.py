@@ -80,8 +80,8 @@ def lib():
     L.bfa_synchronize.argtypes = [_vp]
     L.bfa_time_kernels.argtypes = [_vp, ctypes.c_double, ctypes.c_int, _dp]
     L.bfa_copy_kernel.argtypes = [ctypes.c_size_t, ctypes.c_int, _dp]
-    L.bfa_set_tiles.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
-    L.bfa_get_tiles.argtypes = [_vp] + [ctypes.POINTER(ctypes.c_int)] * 4
+    L.bfa_set_tiles.argtypes = [_vp] + [ctypes.c_int] * 5
+    L.bfa_get_tiles.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
     _lib = L
     return L
 
@@ -197,13 +197,16 @@ class Microenvironment:
         self._check(lib().bfa_get_fp32_basis(self._h, _d(b), _d(s)), "get_fp32_basis")
         return b, s
 
-    def set_tiles(self, Lx=0, Ly=0, Lz=0, rows_x=0):
-        self._check(lib().bfa_set_tiles(self._h, int(Lx), int(Ly), int(Lz), int(rows_x)), "set_tiles")
+    def set_tiles(self, Lx=0, Ly=0, Lz=0, rows_x=0, cols_yz=0):
+        """Kernel shapes (see bfa_set_tiles); 0 means automatic."""
+        self._check(lib().bfa_set_tiles(self._h, int(Lx), int(Ly), int(Lz), int(rows_x),
+                                        int(cols_yz)), "set_tiles")
 
     def tiles(self):
-        v = [ctypes.c_int() for _ in range(4)]
-        self._check(lib().bfa_get_tiles(self._h, *[ctypes.byref(x) for x in v]), "get_tiles")
-        return tuple(x.value for x in v)
+        """(Lx, Ly, Lz, rows_x, cols_yz) currently in use."""
+        v = (ctypes.c_int * 5)()
+        self._check(lib().bfa_get_tiles(self._h, v), "get_tiles")
+        return tuple(v)
 
     # -- field transfer -----------------------------------------------------------
     def upload(self, field, layout="ref"):

@@ -1,0 +1,119 @@
+"""Measured kernel-shape selection for the LOD sweeps.
+
+For one grid shape and precision, times every admissible shape of the three
+sweep kernels on the GPU (HIP events around each launch, via
+``Microenvironment.time_kernels``) and returns the fastest per axis:
+
+* x sweep: the wave-local form with chunk length L in {4, 8, 16, 32}
+  (a row's ceil(nx/L) <= 64 chunks share one wavefront), or the LDS-tiled
+  form with R rows per tile and chunk length L;
+* y and z sweeps: TX in {16, 32, 64} x-columns per block and chunk length L,
+  with TX * ceil(n/L) <= 1024 threads.  TX is shared by y and z; L is chosen
+  per axis.
+
+Results are cached per (nx, ny, nz, ns, precision) in memory and, when a path
+is given, in a JSON file.
+"""
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+
+X_WAVE_L = (4, 8, 16, 32)
+X_LDS = [(R, L) for R in (8, 16, 32) for L in (8, 16, 32)]
+COL_TX = (16, 32, 64)
+COL_L = (4, 8, 16, 32)
+
+_cache: dict = {}
+
+
+def _key(nx, ny, nz, ns, precision):
+    return f"{nx}x{ny}x{nz}x{ns}:{precision}"
+
+
+def _time(env, dt, reps, shape):
+    try:
+        env.set_tiles(*shape)
+    except Exception:
+        return None
+    env.time_kernels(dt, 1)  # first launch of a shape pays code-object load
+    return env.time_kernels(dt, reps)
+
+
+def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=3, cache_path=None,
+             verbose=False):
+    """Return {"Lx", "Ly", "Lz", "rows_x", "cols_yz", "ms": {...}, "trials": [...]}."""
+    key = _key(nx, ny, nz, ns, precision)
+    if key in _cache:
+        return _cache[key]
+    if cache_path and os.path.exists(cache_path):
+        try:
+            doc = json.load(open(cache_path))
+            if key in doc:
+                _cache[key] = doc[key]
+                return doc[key]
+        except Exception:
+            pass
+    rng = np.random.default_rng(0)
+    env = pkg.Microenvironment(nx, ny, nz, ns, 20.0, precision)
+    trials = []
+    try:
+        env.set_coefficients(10 ** rng.uniform(3, 5, ns), rng.uniform(0.01, 1, ns))
+        env.set_dirichlet(np.ones(ns, np.uint8), np.ones(6, np.uint8), np.full((6, ns), 20.0))
+        env.upload(20.0 + rng.standard_normal(nx * ny * nz * ns), "soa")
+        best_x = None
+        for L in X_WAVE_L:
+            ms = _time(env, dt, reps, (L, 0, 0, 0, 0))
+            if ms is not None:
+                trials.append({"axis": "x", "form": "wave", "L": L, "ms": ms[0]})
+                if best_x is None or ms[0] < best_x[0]:
+                    best_x = (ms[0], L, 0)
+        for R, L in X_LDS:
+            ms = _time(env, dt, reps, (L, 0, 0, R, 0))
+            if ms is not None:
+                trials.append({"axis": "x", "form": "lds", "R": R, "L": L, "ms": ms[0]})
+                if best_x is None or ms[0] < best_x[0]:
+                    best_x = (ms[0], L, R)
+        best_col = None
+        for TX in COL_TX:
+            by = bz = None
+            for L in COL_L:
+                ms = _time(env, dt, reps, (best_x[1], L, L, best_x[2], TX))
+                if ms is None:
+                    continue
+                trials.append({"axis": "yz", "TX": TX, "L": L, "ms_y": ms[1], "ms_z": ms[2]})
+                if by is None or ms[1] < by[0]:
+                    by = (ms[1], L)
+                if bz is None or ms[2] < bz[0]:
+                    bz = (ms[2], L)
+            if by is not None and bz is not None:
+                tot = by[0] + bz[0]
+                if best_col is None or tot < best_col[0]:
+                    best_col = (tot, TX, by, bz)
+        res = {"Lx": best_x[1], "rows_x": best_x[2], "cols_yz": best_col[1],
+               "Ly": best_col[2][1], "Lz": best_col[3][1],
+               "ms": {"x": best_x[0], "y": best_col[2][0], "z": best_col[3][0]},
+               "trials": trials}
+    finally:
+        env.close()
+    if verbose:
+        for t in trials:
+            print(t)
+    _cache[key] = res
+    if cache_path:
+        doc = {}
+        if os.path.exists(cache_path):
+            try:
+                doc = json.load(open(cache_path))
+            except Exception:
+                doc = {}
+        doc[key] = {k: v for k, v in res.items() if k != "trials"}
+        with open(cache_path, "w") as f:
+            json.dump(doc, f, indent=1)
+    return res
+
+
+def apply(env, res):
+    env.set_tiles(res["Lx"], res["Ly"], res["Lz"], res["rows_x"], res["cols_yz"])

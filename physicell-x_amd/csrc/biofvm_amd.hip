@@ -81,8 +81,10 @@ struct bfa_env {
 
     int Lreq[3] = {0, 0, 0};
     int Rreq = 0;
+    int TXreq = 0;
     int L[3] = {0, 0, 0};
     int R = 0;
+    int TX[3] = {0, 64, 64};
 
     std::string err;
 };
@@ -143,43 +145,78 @@ int pick_L(int n, int pmax) {
     return 0;
 }
 
-size_t lds_col(const bfa_env* e, int n, int L) {
+size_t lds_col(const bfa_env* e, int n, int L, int TX) {
     const int P = (n + L - 1) / L;
-    return (size_t)(2 * n + 64 * P + P) * e->tsize;
+    return (size_t)(2 * P * L + TX * P + P) * e->tsize;
 }
 
 size_t lds_x(const bfa_env* e, int R, int L) {
     const int P = (e->nx + L - 1) / L;
     const int S = e->nx | 1;
-    return ((size_t)R * S + 2 * e->nx + (size_t)P * R + P) * e->tsize;
+    return ((size_t)R * S + 2 * (size_t)P * L + (size_t)P * R + P) * e->tsize;
 }
 
 int choose_tiles(bfa_env* e) {
+    // y and z sweeps: TX columns per block (32 or 64), P = ceil(n/L) chunk
+    // rows with TX*P <= 1024 threads.
     for (int a = 1; a < 3; ++a) {
         const int n = axis_len(e, a);
-        int L = e->Lreq[a] ? e->Lreq[a] : pick_L(n, 16);
-        if (L == 0 || (n + L - 1) / L > 16)
-            return fail(e, BFA_ERR_ARG, "axis %d of length %d: chunk length %d gives more than 16 chunks",
-                        a, n, L);
-        if (std::find(std::begin(kLs), std::end(kLs), L) == std::end(kLs))
-            return fail(e, BFA_ERR_ARG, "unsupported chunk length %d", L);
-        if (lds_col(e, n, L) > (size_t)e->max_lds)
-            return fail(e, BFA_ERR_ARG, "axis %d: LDS need %zu exceeds %d", a, lds_col(e, n, L),
+        int TX = e->TXreq, L = e->Lreq[a];
+        if (TX == 0 && L == 0) {
+            // keep the register array at 16 nodes (no spills, full occupancy)
+            // and narrow the block when the line needs more chunks
+            const int lmax = 16;
+            TX = 64;
+            L = pick_L(n, 16);
+            if (L == 0 || L > lmax) {
+                TX = 32;
+                L = pick_L(n, 32);
+            }
+            if (L == 0 || L > lmax) {
+                TX = 16;
+                L = pick_L(n, 64);
+            }
+            if (e->prec == BFA_FP64 && L > 8 && (n + 7) / 8 <= 64) {
+                TX = 16;
+                L = 8;
+            }
+        } else if (TX == 0) {
+            const int P = (n + L - 1) / L;
+            TX = P <= 16 ? 64 : (P <= 32 ? 32 : 16);
+        } else if (L == 0) {
+            L = pick_L(n, 1024 / TX);
+        }
+        if (TX != 16 && TX != 32 && TX != 64)
+            return fail(e, BFA_ERR_ARG, "columns per block must be 16, 32 or 64, got %d", TX);
+        if (L == 0 || std::find(std::begin(kLs), std::end(kLs), L) == std::end(kLs))
+            return fail(e, BFA_ERR_ARG, "axis %d of length %d: unsupported chunk length %d", a, n, L);
+        if (((n + L - 1) / L) * TX > 1024)
+            return fail(e, BFA_ERR_ARG, "axis %d of length %d: %d chunks of %d x %d columns exceed 1024 threads",
+                        a, n, (n + L - 1) / L, L, TX);
+        if (lds_col(e, n, L, TX) > (size_t)e->max_lds)
+            return fail(e, BFA_ERR_ARG, "axis %d: LDS need %zu exceeds %d", a, lds_col(e, n, L, TX),
                         e->max_lds);
         e->L[a] = L;
+        e->TX[a] = TX;
     }
-    // x sweep: rows per tile R (power of two), LDS budget 64 KiB unless asked.
+    // x sweep.  Rreq == 0: wave-local kernel (rows of P = ceil(nx/L) <= 64
+    // chunks inside one wavefront).  Rreq >= 1: LDS-tiled kernel with Rreq rows.
     const int nx = e->nx;
-    int R = e->Rreq;
-    if (R == 0) {
-        R = 64;
-        const size_t budget = std::min<size_t>(65536, (size_t)e->max_lds);
-        while (R > 1) {
-            const int L = e->Lreq[0] ? e->Lreq[0] : pick_L(nx, std::max(1, 1024 / R));
-            if (L && lds_x(e, R, L) <= budget && ((nx + L - 1) / L) * R <= 1024) break;
-            R >>= 1;
+    if (e->Rreq == 0) {
+        int L = e->Lreq[0];
+        if (L == 0) {
+            L = e->prec == BFA_FP32 ? 16 : 8;
+            while (L < 64 && (nx + L - 1) / L > 64) L <<= 1;
         }
+        if (std::find(std::begin(kLs), std::end(kLs), L) == std::end(kLs) || L > 32)
+            return fail(e, BFA_ERR_ARG, "wave x sweep: unsupported chunk length %d", L);
+        if ((nx + L - 1) / L > 64)
+            return fail(e, BFA_ERR_ARG, "wave x sweep: nx=%d needs more than 64 chunks of %d", nx, L);
+        e->L[0] = L;
+        e->R = 0;
+        return BFA_OK;
     }
+    int R = e->Rreq;
     if (R < 1 || R > 64 || (R & (R - 1)))
         return fail(e, BFA_ERR_ARG, "x rows per tile must be a power of two in [1,64], got %d", R);
     int L = e->Lreq[0] ? e->Lreq[0] : pick_L(nx, std::max(1, 1024 / R));
@@ -247,16 +284,46 @@ int launch_x_L(bfa_env* e) {
     return BFA_OK;
 }
 
+template <typename T, int L>
+int launch_xw_L(bfa_env* e) {
+    constexpr bool DEV = std::is_same<T, float>::value;
+    const int nx = e->nx;
+    const int P = (nx + L - 1) / L;
+    const int RW = 64 / P;
+    const long long rows = (long long)e->ny * e->nz;
+    const long long groups = (rows + RW - 1) / RW;
+    const long long want = std::max<long long>(1, 2048 / e->ns);
+    const unsigned bx = (unsigned)std::max<long long>(1, std::min<long long>((groups + 3) / 4, want));
+    const bool vec = ((size_t)nx * sizeof(T)) % 16 == 0 && ((size_t)L * sizeof(T)) % 16 == 0;
+    dim3 grid(bx, (unsigned)e->ns);
+    if (vec)
+        hipLaunchKernelGGL((bfa::k_sweep_x_wave<T, L, DEV, true>), grid, dim3(256), 0, e->stream,
+                           sweep_args<T>(e, 0));
+    else
+        hipLaunchKernelGGL((bfa::k_sweep_x_wave<T, L, DEV, false>), grid, dim3(256), 0, e->stream,
+                           sweep_args<T>(e, 0));
+    LAUNCH_CHECK(e, "x sweep (wave)");
+    return BFA_OK;
+}
+
 template <typename T, int L, int AXIS>
 int launch_col_L(bfa_env* e) {
     constexpr bool DEV = std::is_same<T, float>::value;
     const int n = axis_len(e, AXIS);
     const int P = (n + L - 1) / L;
+    const int TX = e->TX[AXIS];
     const long long ncols = AXIS == 1 ? (long long)e->nz * e->nx : (long long)e->ny * e->nx;
-    const size_t lds = lds_col(e, n, L);
-    dim3 grid((unsigned)((ncols + 63) / 64), (unsigned)e->ns);
-    auto fn = bfa::k_sweep_col<T, L, AXIS, DEV>;
-    hipLaunchKernelGGL(fn, grid, dim3(64 * P), lds, e->stream, sweep_args<T>(e, AXIS));
+    const size_t lds = lds_col(e, n, L, TX);
+    dim3 grid((unsigned)((ncols + TX - 1) / TX), (unsigned)e->ns);
+    if (TX == 64)
+        hipLaunchKernelGGL((bfa::k_sweep_col<T, L, AXIS, DEV, 64>), grid, dim3(64 * P), lds,
+                           e->stream, sweep_args<T>(e, AXIS));
+    else if (TX == 32)
+        hipLaunchKernelGGL((bfa::k_sweep_col<T, L, AXIS, DEV, 32>), grid, dim3(32 * P), lds,
+                           e->stream, sweep_args<T>(e, AXIS));
+    else
+        hipLaunchKernelGGL((bfa::k_sweep_col<T, L, AXIS, DEV, 16>), grid, dim3(16 * P), lds,
+                           e->stream, sweep_args<T>(e, AXIS));
     LAUNCH_CHECK(e, AXIS == 1 ? "y sweep" : "z sweep");
     return BFA_OK;
 }
@@ -269,14 +336,19 @@ int raise_lds_limits(bfa_env* e) {
     const void* fx = nullptr;
     const void* fy = nullptr;
     const void* fz = nullptr;
-#define BFA_PICK(Lv)                                                 \
-    if (e->L[0] == Lv) fx = (const void*)bfa::k_sweep_x<T, Lv, DEV>;    \
-    if (e->L[1] == Lv) fy = (const void*)bfa::k_sweep_col<T, Lv, 1, DEV>; \
-    if (e->L[2] == Lv) fz = (const void*)bfa::k_sweep_col<T, Lv, 2, DEV>;
+#define BFA_COLFN(Lv, A)                                                       \
+    (e->TX[A] == 64   ? (const void*)bfa::k_sweep_col<T, Lv, A, DEV, 64>       \
+     : e->TX[A] == 32 ? (const void*)bfa::k_sweep_col<T, Lv, A, DEV, 32>       \
+                      : (const void*)bfa::k_sweep_col<T, Lv, A, DEV, 16>)
+#define BFA_PICK(Lv)                                                          \
+    if (e->L[0] == Lv && e->R) fx = (const void*)bfa::k_sweep_x<T, Lv, DEV>;  \
+    if (e->L[1] == Lv) fy = BFA_COLFN(Lv, 1);                                 \
+    if (e->L[2] == Lv) fz = BFA_COLFN(Lv, 2);
     BFA_PICK(4) BFA_PICK(8) BFA_PICK(16) BFA_PICK(32) BFA_PICK(64)
 #undef BFA_PICK
-    const size_t need[3] = {lds_x(e, e->R, e->L[0]), lds_col(e, e->ny, e->L[1]),
-                            lds_col(e, e->nz, e->L[2])};
+#undef BFA_COLFN
+    const size_t need[3] = {e->R ? lds_x(e, e->R, e->L[0]) : 0, lds_col(e, e->ny, e->L[1], e->TX[1]),
+                            lds_col(e, e->nz, e->L[2], e->TX[2])};
     const void* fn[3] = {fx, fy, fz};
     for (int a = 0; a < 3; ++a)
         if (need[a] > 65536 && fn[a])
@@ -288,7 +360,14 @@ int raise_lds_limits(bfa_env* e) {
 template <typename T>
 int launch_sweep(bfa_env* e, int axis) {
     const int L = e->L[axis];
-    if (axis == 0) {
+    if (axis == 0 && e->R == 0) {
+        switch (L) {
+            case 4: return launch_xw_L<T, 4>(e);
+            case 8: return launch_xw_L<T, 8>(e);
+            case 16: return launch_xw_L<T, 16>(e);
+            case 32: return launch_xw_L<T, 32>(e);
+        }
+    } else if (axis == 0) {
         switch (L) {
             case 4: return launch_x_L<T, 4>(e);
             case 8: return launch_x_L<T, 8>(e);
@@ -1010,36 +1089,39 @@ int bfa_copy_kernel(size_t bytes, int reps, double* ms) {
     return BFA_OK;
 }
 
-int bfa_set_tiles(bfa_env* e, int Lx, int Ly, int Lz, int rows_x) {
+int bfa_set_tiles(bfa_env* e, int Lx, int Ly, int Lz, int rows_x, int cols_yz) {
     if (!e) return BFA_ERR_ARG;
-    const int save[4] = {e->Lreq[0], e->Lreq[1], e->Lreq[2], e->Rreq};
-    const int saveL[4] = {e->L[0], e->L[1], e->L[2], e->R};
+    int sLreq[3], sL[3], sTX[3];
+    std::copy(e->Lreq, e->Lreq + 3, sLreq);
+    std::copy(e->L, e->L + 3, sL);
+    std::copy(e->TX, e->TX + 3, sTX);
+    const int sRreq = e->Rreq, sR = e->R, sTXreq = e->TXreq;
     e->Lreq[0] = Lx;
     e->Lreq[1] = Ly;
     e->Lreq[2] = Lz;
     e->Rreq = rows_x;
+    e->TXreq = cols_yz;
     int rc = apply_tiles(e);
     if (rc) {
-        e->Lreq[0] = save[0];
-        e->Lreq[1] = save[1];
-        e->Lreq[2] = save[2];
-        e->Rreq = save[3];
-        e->L[0] = saveL[0];
-        e->L[1] = saveL[1];
-        e->L[2] = saveL[2];
-        e->R = saveL[3];
+        std::copy(sLreq, sLreq + 3, e->Lreq);
+        std::copy(sL, sL + 3, e->L);
+        std::copy(sTX, sTX + 3, e->TX);
+        e->Rreq = sRreq;
+        e->R = sR;
+        e->TXreq = sTXreq;
         return rc;
     }
     invalidate_graph(e);
     return BFA_OK;
 }
 
-int bfa_get_tiles(const bfa_env* e, int* Lx, int* Ly, int* Lz, int* rows_x) {
-    if (!e || !Lx || !Ly || !Lz || !rows_x) return BFA_ERR_ARG;
-    *Lx = e->L[0];
-    *Ly = e->L[1];
-    *Lz = e->L[2];
-    *rows_x = e->R;
+int bfa_get_tiles(const bfa_env* e, int* out5) {
+    if (!e || !out5) return BFA_ERR_ARG;
+    out5[0] = e->L[0];
+    out5[1] = e->L[1];
+    out5[2] = e->L[2];
+    out5[3] = e->R;
+    out5[4] = e->TX[1] == e->TX[2] ? e->TX[1] : -1;
     return BFA_OK;
 }
 

@@ -62,61 +62,55 @@ struct SweepArgs {
     unsigned reg;     // bit f set: face f registered
 };
 
-// ---- chunk recurrences (positions s..s+L-1 of a line of length n) ----------
+// ---- chunk recurrences (positions s..s+L-1 of a padded line) ----------------
+// Coefficient arrays are padded to P*L entries with 1/den = c' = 0 past the
+// line end, and padded values are 0, so the recurrences need no masking: the
+// forward pass carries zeros through the padding and the backward pass starts
+// from zero at the true line end.
 
 template <typename T, int L>
-__device__ __forceinline__ void fwd_local(const T (&x)[L], int s, int n,
-                                          const T* __restrict__ inv, T k, T& zend, T& G) {
+__device__ __forceinline__ void fwd_local(const T (&x)[L], int s, const T* __restrict__ inv, T k,
+                                          T& zend, T& G) {
     T z = T(0), g = T(1);
 #pragma unroll
     for (int m = 0; m < L; ++m) {
-        if (s + m < n) {
-            const T iv = inv[s + m];
-            z = fma(k, z, x[m]) * iv;
-            g = g * (k * iv);
-        }
+        const T iv = inv[s + m];
+        z = fma(k, z, x[m]) * iv;
+        g = g * (k * iv);
     }
     zend = z;
     G = g;
 }
 
 template <typename T, int L>
-__device__ __forceinline__ void fwd_apply(T (&x)[L], int s, int n, const T* __restrict__ inv,
-                                          T k, T y) {
+__device__ __forceinline__ void fwd_apply(T (&x)[L], int s, const T* __restrict__ inv, T k, T y) {
 #pragma unroll
     for (int m = 0; m < L; ++m) {
-        if (s + m < n) {
-            y = fma(k, y, x[m]) * inv[s + m];
-            x[m] = y;
-        }
+        y = fma(k, y, x[m]) * inv[s + m];
+        x[m] = y;
     }
 }
 
 template <typename T, int L>
-__device__ __forceinline__ void bwd_local(const T (&x)[L], int s, int n,
-                                          const T* __restrict__ cp, T& wstart, T& H) {
+__device__ __forceinline__ void bwd_local(const T (&x)[L], int s, const T* __restrict__ cp,
+                                          T& wstart, T& H) {
     T w = T(0), h = T(1);
 #pragma unroll
     for (int m = L - 1; m >= 0; --m) {
-        if (s + m < n) {
-            const T c = cp[s + m];
-            w = fma(-c, w, x[m]);
-            h = h * (-c);
-        }
+        const T c = cp[s + m];
+        w = fma(-c, w, x[m]);
+        h = h * (-c);
     }
     wstart = w;
     H = h;
 }
 
 template <typename T, int L>
-__device__ __forceinline__ void bwd_apply(T (&x)[L], int s, int n, const T* __restrict__ cp,
-                                          T xn) {
+__device__ __forceinline__ void bwd_apply(T (&x)[L], int s, const T* __restrict__ cp, T xn) {
 #pragma unroll
     for (int m = L - 1; m >= 0; --m) {
-        if (s + m < n) {
-            xn = fma(-cp[s + m], xn, x[m]);
-            x[m] = xn;
-        }
+        xn = fma(-cp[s + m], xn, x[m]);
+        x[m] = xn;
     }
 }
 
@@ -126,12 +120,12 @@ __device__ __forceinline__ void bwd_apply(T (&x)[L], int s, int n, const T* __re
 // with act == false only take part in the barriers.
 template <typename T, int L>
 __device__ __forceinline__ void chunk_solve(T (&x)[L], bool act, int p, int P, int line,
-                                            int lines, int n, const T* __restrict__ s_inv,
+                                            int lines, const T* __restrict__ s_inv,
                                             const T* __restrict__ s_cp, T k, T* s_z, T* s_g) {
     const int s = p * L;
     T a = T(0), g = T(1);
     if (act) {
-        fwd_local<T, L>(x, s, n, s_inv, k, a, g);
+        fwd_local<T, L>(x, s, s_inv, k, a, g);
         s_z[p * lines + line] = a;
         if (line == 0) s_g[p] = g;
     }
@@ -139,11 +133,11 @@ __device__ __forceinline__ void chunk_solve(T (&x)[L], bool act, int p, int P, i
     if (act) {
         T y = T(0);
         for (int q = 0; q < p; ++q) y = fma(s_g[q], y, s_z[q * lines + line]);
-        fwd_apply<T, L>(x, s, n, s_inv, k, y);
+        fwd_apply<T, L>(x, s, s_inv, k, y);
     }
     __syncthreads();
     if (act) {
-        bwd_local<T, L>(x, s, n, s_cp, a, g);
+        bwd_local<T, L>(x, s, s_cp, a, g);
         s_z[p * lines + line] = a;
         if (line == 0) s_g[p] = g;
     }
@@ -151,13 +145,24 @@ __device__ __forceinline__ void chunk_solve(T (&x)[L], bool act, int p, int P, i
     if (act) {
         T xn = T(0);
         for (int q = P - 1; q > p; --q) xn = fma(s_g[q], xn, s_z[q * lines + line]);
-        bwd_apply<T, L>(x, s, n, s_cp, xn);
+        bwd_apply<T, L>(x, s, s_cp, xn);
+    }
+}
+
+// Stage padded coefficients of substrate d into LDS: s_inv/s_cp[0..PL).
+template <typename T>
+__device__ __forceinline__ void stage_coefs(T* s_inv, T* s_cp, const T* __restrict__ inv,
+                                            const T* __restrict__ cp, int n, int PL, int t,
+                                            int nthr) {
+    for (int i = t; i < PL; i += nthr) {
+        s_inv[i] = i < n ? inv[i] : T(0);
+        s_cp[i] = i < n ? cp[i] : T(0);
     }
 }
 
 // ---- x sweep: R rows per block staged through LDS ---------------------------
 // blockDim.x = roundup(R*P, 64) with P = ceil(nx/L); grid = (ceil(ny*nz/R), ns).
-// Dynamic LDS: (R*S + 2*nx + P*R + P) * sizeof(T), S = nx | 1.
+// Dynamic LDS: (R*S + 2*P*L + P*R + P) * sizeof(T), S = nx | 1.
 template <typename T, int L, bool DEV>
 __global__ __launch_bounds__(1024) void k_sweep_x(SweepArgs<T> a, int R, int logR) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -166,8 +171,8 @@ __global__ __launch_bounds__(1024) void k_sweep_x(SweepArgs<T> a, int R, int log
     const int S = nx | 1;
     T* tile = reinterpret_cast<T*>(smem);
     T* s_inv = tile + (size_t)R * S;
-    T* s_cp = s_inv + nx;
-    T* s_z = s_cp + nx;
+    T* s_cp = s_inv + P * L;
+    T* s_z = s_cp + P * L;
     T* s_g = s_z + (size_t)P * R;
 
     const int t = threadIdx.x;
@@ -178,10 +183,8 @@ __global__ __launch_bounds__(1024) void k_sweep_x(SweepArgs<T> a, int R, int log
     const int rvalid = (int)((rows - r0) < R ? (rows - r0) : R);
     T* g = a.u + (long long)d * a.N + r0 * nx;
 
-    for (int i = t; i < nx; i += nthr) {
-        s_inv[i] = a.inv[(long long)d * nx + i];
-        s_cp[i] = a.cp[(long long)d * nx + i];
-    }
+    stage_coefs<T>(s_inv, s_cp, a.inv + (long long)d * nx, a.cp + (long long)d * nx, nx, P * L, t,
+                   nthr);
     const int wave = t >> 6, lane = t & 63, nw = nthr >> 6;
     for (int row = wave; row < rvalid; row += nw) {
         const T* src = g + (long long)row * nx;
@@ -217,9 +220,9 @@ __global__ __launch_bounds__(1024) void k_sweep_x(SweepArgs<T> a, int R, int log
     if (DEV) {
         const T bt = a.beta[d];
 #pragma unroll
-        for (int m = 0; m < L; ++m) x[m] -= bt;
+        for (int m = 0; m < L; ++m) x[m] = (s + m < nx) ? x[m] - bt : T(0);
     }
-    chunk_solve<T, L>(x, act, p, P, rho, R, nx, s_inv, s_cp, a.k[d], s_z, s_g);
+    chunk_solve<T, L>(x, act, p, P, rho, R, s_inv, s_cp, a.k[d], s_z, s_g);
     if (act && dact) {
 #pragma unroll
         for (int m = 0; m < L; ++m) {
@@ -241,31 +244,211 @@ __global__ __launch_bounds__(1024) void k_sweep_x(SweepArgs<T> a, int R, int log
     }
 }
 
-// ---- y / z sweeps: one lane per x-column, 64 columns per block --------------
+// ---- x sweep, wave-local variant -------------------------------------------
+// A row of nx nodes is cut into P = ceil(nx/L) chunks owned by P consecutive
+// lanes of one wavefront, so a wave holds RW = 64/P whole rows and never
+// synchronises with other waves.  Each lane reads its L contiguous nodes with
+// 16-byte vector loads when the row pitch allows (VEC), the chunk carries are
+// combined with a Hillis-Steele scan of affine maps y -> a*y + b over the
+// lanes of the row (shuffles, log2(P) steps each way), and rows are
+// prefetched one group ahead so that loads of group g+1 overlap the solve of
+// group g.  The lane's 1/den and c' values stay in registers for the whole
+// kernel.  Grid: (blocks, ns) with 256-thread blocks; each wave walks row
+// groups g = wave_id, wave_id + total_waves, ...
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    using type = f32x4;
+    static constexpr int n = 4;
+};
+template <>
+struct Vec16<double> {
+    using type = f64x2;
+    static constexpr int n = 2;
+};
+
+template <typename T, int L, bool VEC>
+__device__ __forceinline__ void load_chunk(T (&x)[L], const T* __restrict__ src, int valid) {
+    if (VEC && valid == L) {
+        using V = typename Vec16<T>::type;
+        constexpr int W = Vec16<T>::n;
+        const V* v = reinterpret_cast<const V*>(src);
+#pragma unroll
+        for (int q = 0; q < L / W; ++q) {
+            const V t = __builtin_nontemporal_load(v + q);
+#pragma unroll
+            for (int w = 0; w < W; ++w) x[q * W + w] = t[w];
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < L; ++m) x[m] = m < valid ? src[m] : T(0);
+    }
+}
+
+template <typename T, int L, bool VEC>
+__device__ __forceinline__ void store_chunk(const T (&x)[L], T* __restrict__ dst, int valid) {
+    if (VEC && valid == L) {
+        using V = typename Vec16<T>::type;
+        constexpr int W = Vec16<T>::n;
+        V* v = reinterpret_cast<V*>(dst);
+#pragma unroll
+        for (int q = 0; q < L / W; ++q) {
+            V t;
+#pragma unroll
+            for (int w = 0; w < W; ++w) t[w] = x[q * W + w];
+            __builtin_nontemporal_store(t, v + q);
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < L; ++m)
+            if (m < valid) dst[m] = x[m];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T shfl_up_t(T v, int d) { return __shfl_up(v, d, 64); }
+template <typename T>
+__device__ __forceinline__ T shfl_down_t(T v, int d) { return __shfl_down(v, d, 64); }
+
+template <typename T, int L, bool DEV, bool VEC>
+__global__ __launch_bounds__(256) void k_sweep_x_wave(SweepArgs<T> a) {
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int P = (nx + L - 1) / L;
+    const int RW = 64 / P;
+    const int lane = threadIdx.x & 63;
+    const int rw = lane / P;
+    const int p = lane - rw * P;
+    const bool lane_on = rw < RW;
+    const int d = blockIdx.y;
+    const int s = p * L;
+    const int valid = lane_on ? (nx - s < L ? nx - s : L) : 0;
+    const int rows = ny * nz;
+    const int ngroups = (rows + RW - 1) / RW;
+    const int wstride = gridDim.x * (blockDim.x >> 6);
+    const T k = a.k[d];
+    const T bt = DEV ? a.beta[d] : T(0);
+    const bool dact = (a.active >> d) & 1ull;
+    // Nodes past the row end get 1/den = c' = 0 and value 0: the recurrences
+    // then carry zeros through them and need no per-node masking.
+    T inv[L], cp[L];
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+        inv[m] = m < valid ? a.inv[d * nx + s + m] : T(0);
+        cp[m] = m < valid ? a.cp[d * nx + s + m] : T(0);
+    }
+    T* ub = a.u + (long long)d * a.N;
+    for (int g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); g < ngroups; g += wstride) {
+        const int r = g * RW + rw;
+        const bool have = lane_on && r < rows;
+        T cur[L];
+        if (have)
+            load_chunk<T, L, VEC>(cur, ub + r * nx + s, valid);
+        else {
+#pragma unroll
+            for (int m = 0; m < L; ++m) cur[m] = T(0);
+        }
+        int jj = 0, kk = 0;
+        bool rowb = false;
+        if (have) {
+            kk = r / ny;
+            jj = r - kk * ny;
+            rowb = (jj == 0 || jj == ny - 1 || kk == 0 || kk == nz - 1);
+            if (dact && (rowb || s == 0 || s + valid == nx)) {
+                for (int m = 0; m < valid; ++m) {
+                    const int pos = s + m;
+                    if (rowb || pos == 0 || pos == nx - 1)
+                        cur[m] = a.dval[dir_slot(pos, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
+                }
+            }
+        }
+        if (DEV) {
+#pragma unroll
+            for (int m = 0; m < L; ++m) cur[m] = (m < valid) ? cur[m] - bt : T(0);
+        }
+        // forward: local zero-carry pass
+        T za = T(1), zb = T(0);
+#pragma unroll
+        for (int m = 0; m < L; ++m) {
+            zb = fma(k, zb, cur[m]) * inv[m];
+            za = za * (k * inv[m]);
+        }
+        // inclusive scan of y -> za*y + zb over the lanes of this row
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const T pa = shfl_up_t(za, off), pb = shfl_up_t(zb, off);
+            if (off < P && p >= off) {
+                zb = fma(za, pb, zb);
+                za = za * pa;
+            }
+        }
+        T y = shfl_up_t(zb, 1);
+        if (p == 0) y = T(0);
+#pragma unroll
+        for (int m = 0; m < L; ++m) {
+            y = fma(k, y, cur[m]) * inv[m];
+            cur[m] = y;
+        }
+        // backward: local zero-carry pass, then suffix scan
+        T wa = T(1), wb = T(0);
+#pragma unroll
+        for (int m = L - 1; m >= 0; --m) {
+            wb = fma(-cp[m], wb, cur[m]);
+            wa = wa * (-cp[m]);
+        }
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const T na = shfl_down_t(wa, off), nb = shfl_down_t(wb, off);
+            if (off < P && p + off < P) {
+                wb = fma(wa, nb, wb);
+                wa = wa * na;
+            }
+        }
+        T xn = shfl_down_t(wb, 1);
+        if (p == P - 1) xn = T(0);
+#pragma unroll
+        for (int m = L - 1; m >= 0; --m) {
+            xn = fma(-cp[m], xn, cur[m]);
+            cur[m] = xn;
+        }
+        if (have) {
+            if (dact && (rowb || s == 0 || s + valid == nx)) {
+                for (int m = 0; m < valid; ++m) {
+                    const int pos = s + m;
+                    if (rowb || pos == 0 || pos == nx - 1)
+                        cur[m] = a.dval[dir_slot(pos, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
+                }
+            }
+            store_chunk<T, L, VEC>(cur, ub + r * nx + s, valid);
+        }
+    }
+}
+
+// ---- y / z sweeps: one lane per x-column, TX columns per block -------------
 // AXIS 1 = y (line stride nx, columns (k,i)), AXIS 2 = z (stride nx*ny,
-// columns (j,i)).  blockDim.x = 64*P with P = ceil(n/L);
-// grid = (ceil(ncols/64), ns).  Dynamic LDS: (2*n + 64*P + P) * sizeof(T).
-template <typename T, int L, int AXIS, bool DEV>
+// columns (j,i)).  blockDim.x = TX*P with P = ceil(n/L);
+// grid = (ceil(ncols/TX), ns).  Dynamic LDS: (2*P*L + TX*P + P) * sizeof(T).
+template <typename T, int L, int AXIS, bool DEV, int TX>
 __global__ __launch_bounds__(1024) void k_sweep_col(SweepArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int n = AXIS == 1 ? ny : nz;
     const int P = (n + L - 1) / L;
     T* s_inv = reinterpret_cast<T*>(smem);
-    T* s_cp = s_inv + n;
-    T* s_z = s_cp + n;
-    T* s_g = s_z + (size_t)64 * P;
+    T* s_cp = s_inv + P * L;
+    T* s_z = s_cp + P * L;
+    T* s_g = s_z + (size_t)TX * P;
 
     const int t = threadIdx.x;
-    const int lane = t & 63;
-    const int p = t >> 6;
+    const int lane = t % TX;
+    const int p = t / TX;
     const int d = blockIdx.y;
-    for (int i = t; i < n; i += blockDim.x) {
-        s_inv[i] = a.inv[(long long)d * n + i];
-        s_cp[i] = a.cp[(long long)d * n + i];
-    }
+    stage_coefs<T>(s_inv, s_cp, a.inv + (long long)d * n, a.cp + (long long)d * n, n, P * L, t,
+                   blockDim.x);
     const long long ncols = AXIS == 1 ? (long long)nz * nx : (long long)ny * nx;
-    const long long c = (long long)blockIdx.x * 64 + lane;
+    const long long c = (long long)blockIdx.x * TX + lane;
     const bool act = c < ncols;
     long long base = 0, stride;
     int ci = 0, co = 0;  // i and the other transverse coordinate (k for y, j for z)
@@ -287,10 +470,10 @@ __global__ __launch_bounds__(1024) void k_sweep_col(SweepArgs<T> a) {
     if (DEV) {
         const T bt = a.beta[d];
 #pragma unroll
-        for (int m = 0; m < L; ++m) x[m] -= bt;
+        for (int m = 0; m < L; ++m) x[m] = (s + m < n) ? x[m] - bt : T(0);
     }
     __syncthreads();  // coefficient staging complete
-    chunk_solve<T, L>(x, act, p, P, lane, 64, n, s_inv, s_cp, a.k[d], s_z, s_g);
+    chunk_solve<T, L>(x, act, p, P, lane, TX, s_inv, s_cp, a.k[d], s_z, s_g);
     const bool dact = (a.active >> d) & 1ull;
     if (act && dact) {
         const bool lineb = AXIS == 1 ? (ci == 0 || ci == nx - 1 || co == 0 || co == nz - 1)
