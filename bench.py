@@ -85,6 +85,7 @@ def make_env(pkg, w, n, ns, precision, tiles=None):
     env = pkg.Microenvironment(n, n, n, ns, 20.0, precision)
     if tiles is not None:
         env.set_tiles(tiles["Lx"], tiles["Ly"], tiles["Lz"], tiles["rows_x"], tiles["cols_yz"])
+        env.set_slab(tiles.get("slab", 0))
     env.set_coefficients(w["D"], w["decay"])
     env.set_dirichlet(np.ones(ns, np.uint8), np.ones(6, np.uint8), np.tile(w["dval"], (6, 1)))
     env.upload(w["field"], "soa")
@@ -205,9 +206,11 @@ def main():
         env64.close()
     env32.close()
 
-    names = ["x", "y", "z"]
+    slab32 = tiles["fp32"]["slab"] if tiles["fp32"] else 0
+    names = ["xy_slab" if slab32 else "x", "y", "z"]
     dom = int(np.argmax(ms32[:3]))
-    alg_bytes = updates * 2 * 4  # one FP32 read + one write per voxel-substrate per sweep
+    sweeps = 2 if (slab32 and dom == 0) else 1
+    alg_bytes = updates * 2 * 4 * sweeps  # one FP32 read + one write per voxel-substrate per sweep
     achieved = alg_bytes / (ms32[dom] * 1e-3) / 1e9
     traffic = load_traffic(f"sweep_{names[dom]}_fp32_{n}")
     out = {
@@ -226,7 +229,8 @@ def main():
         "config": {"workload": f"{n}^3 voxels x {ns} substrates, {args.agents} agents, dt=0.01 min, "
                                "FP32 scaled-deviation LOD step + agents",
                    "grid": [n, n, n], "substrates": ns, "agents": args.agents, "seed": args.seed},
-        "roofline": {"bound": "hbm", "kernel": f"sweep_{names[dom]}", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": f"sweep_{names[dom]}", "sweeps_in_kernel": sweeps,
+                     "alg_bytes_per_launch": alg_bytes, "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": traffic},
         "kernel_ms": dict(x=ms32[0], y=ms32[1], z=ms32[2], agents=ms32[3]),
@@ -234,7 +238,7 @@ def main():
         "copy_kernel_GBps": copy_gbs,
         "graph_ms_per_step": 1e3 * gwall32 / args.steps,
         "tiles": dict(Lx=tiles32[0], Ly=tiles32[1], Lz=tiles32[2], rows_x=tiles32[3],
-                      cols_yz=tiles32[4], autotuned=not args.no_tune),
+                      cols_yz=tiles32[4], slab=slab32, autotuned=not args.no_tune),
     }
     out.update(res)
     if not args.no_cpu:

@@ -144,6 +144,16 @@ int bfa_copy_kernel(size_t bytes, int reps, double *ms);
 int bfa_set_tiles(bfa_env *env, int Lx, int Ly, int Lz, int rows_x, int cols_yz);
 int bfa_get_tiles(const bfa_env *env, int *out5);
 
+/* Sweep schedule.  planes == 0: each sweep covers the whole volume.
+ * planes > 0: the x and y sweeps run in slabs of `planes` k-planes per
+ * substrate, the y sweep of a slab directly after its x sweep (it re-reads
+ * the slab from cache), on two streams that overlap neighbouring slabs; the
+ * z sweep then covers the volume.  Needs the wave-local x sweep.  In this
+ * schedule bfa_time_kernels reports the x+y slab phase in ms[0] and 0 in
+ * ms[1]. */
+int bfa_set_slab(bfa_env *env, int planes);
+int bfa_get_slab(const bfa_env *env);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,6 +82,8 @@ def lib():
     L.bfa_copy_kernel.argtypes = [ctypes.c_size_t, ctypes.c_int, _dp]
     L.bfa_set_tiles.argtypes = [_vp] + [ctypes.c_int] * 5
     L.bfa_get_tiles.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
+    L.bfa_set_slab.argtypes = [_vp, ctypes.c_int]
+    L.bfa_get_slab.argtypes = [_vp]
     _lib = L
     return L
 
@@ -207,6 +209,13 @@ class Microenvironment:
         v = (ctypes.c_int * 5)()
         self._check(lib().bfa_get_tiles(self._h, v), "get_tiles")
         return tuple(v)
+
+    def set_slab(self, planes):
+        """x/y sweeps in k-slabs of `planes` planes (0: whole-volume sweeps)."""
+        self._check(lib().bfa_set_slab(self._h, int(planes)), "set_slab")
+
+    def slab(self):
+        return lib().bfa_get_slab(self._h)
 
     # -- field transfer -----------------------------------------------------------
     def upload(self, field, layout="ref"):

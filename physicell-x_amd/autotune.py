@@ -9,7 +9,9 @@ sweep kernels on the GPU (HIP events around each launch, via
   form with R rows per tile and chunk length L;
 * y and z sweeps: TX in {16, 32, 64} x-columns per block and chunk length L,
   with TX * ceil(n/L) <= 1024 threads.  TX is shared by y and z; L is chosen
-  per axis.
+  per axis;
+* schedule: whole-volume sweeps, or x/y sweeps in k-slabs of a few planes
+  (chosen by whole-step time).
 
 Results are cached per (nx, ny, nz, ns, precision) in memory and, when a path
 is given, in a JSON file.
@@ -25,6 +27,7 @@ X_WAVE_L = (4, 8, 16, 32)
 X_LDS = [(R, L) for R in (8, 16, 32) for L in (8, 16, 32)]
 COL_TX = (16, 32, 64)
 COL_L = (4, 8, 16, 32)
+SLAB_BYTES = (8 << 20, 16 << 20, 32 << 20, 64 << 20, 128 << 20)
 
 _cache: dict = {}
 
@@ -92,9 +95,29 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=3, cache_path=
                 tot = by[0] + bz[0]
                 if best_col is None or tot < best_col[0]:
                     best_col = (tot, TX, by, bz)
+        shape = (best_x[1], best_col[2][1], best_col[3][1], best_x[2], best_col[1])
+        env.set_tiles(*shape)
+        env.time_kernels(dt, 1)
+        step0 = env.time_kernels(dt, reps)[4]
+        best_slab = (step0, 0)
+        trials.append({"axis": "schedule", "slab": 0, "ms_step": step0})
+        if best_x[2] == 0:
+            plane = nx * ny * (4 if precision in ("fp32", "f32", 1) else 8)
+            for nbytes in SLAB_BYTES:
+                planes = max(1, nbytes // plane)
+                if planes >= nz:
+                    continue
+                env.set_slab(planes)
+                env.time_kernels(dt, 1)
+                t = env.time_kernels(dt, reps)[4]
+                trials.append({"axis": "schedule", "slab": planes, "ms_step": t})
+                if t < best_slab[0]:
+                    best_slab = (t, planes)
+            env.set_slab(0)
         res = {"Lx": best_x[1], "rows_x": best_x[2], "cols_yz": best_col[1],
-               "Ly": best_col[2][1], "Lz": best_col[3][1],
-               "ms": {"x": best_x[0], "y": best_col[2][0], "z": best_col[3][0]},
+               "Ly": best_col[2][1], "Lz": best_col[3][1], "slab": best_slab[1],
+               "ms": {"x": best_x[0], "y": best_col[2][0], "z": best_col[3][0],
+                      "step": best_slab[0]},
                "trials": trials}
     finally:
         env.close()
@@ -117,3 +140,4 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=3, cache_path=
 
 def apply(env, res):
     env.set_tiles(res["Lx"], res["Ly"], res["Lz"], res["rows_x"], res["cols_yz"])
+    env.set_slab(res.get("slab", 0))

@@ -85,6 +85,11 @@ struct bfa_env {
     int L[3] = {0, 0, 0};
     int R = 0;
     int TX[3] = {0, 64, 64};
+    int slab = 0;  // k-planes per x/y slab pair (0: whole-volume sweeps)
+
+    hipStream_t aux[2] = {nullptr, nullptr};
+    hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_join[2] = {nullptr, nullptr};
 
     std::string err;
 };
@@ -242,6 +247,19 @@ int apply_tiles(bfa_env* e) {
     return e->prec == BFA_FP32 ? raise_lds_limits<float>(e) : raise_lds_limits<double>(e);
 }
 
+// A launch window: substrates [d0, d0+nd), lines [first, first+count).
+struct Win {
+    int d0, nd;
+    long long first, count;
+};
+
+long long axis_lines(const bfa_env* e, int axis) {
+    return axis == 0 ? (long long)e->ny * e->nz
+                     : (axis == 1 ? (long long)e->nz * e->nx : (long long)e->ny * e->nx);
+}
+
+Win full_win(const bfa_env* e, int axis) { return Win{0, e->ns, 0, axis_lines(e, axis)}; }
+
 template <typename T>
 bfa::SweepArgs<T> sweep_args(const bfa_env* e, int axis) {
     bfa::SweepArgs<T> a;
@@ -264,11 +282,15 @@ bfa::SweepArgs<T> sweep_args(const bfa_env* e, int axis) {
     a.ny = e->ny;
     a.nz = e->nz;
     a.ns = e->ns;
+    a.d0 = 0;
+    a.first = 0;
+    a.count = axis == 0 ? (long long)e->ny * e->nz
+                        : (axis == 1 ? (long long)e->nz * e->nx : (long long)e->ny * e->nx);
     return a;
 }
 
 template <typename T, int L>
-int launch_x_L(bfa_env* e) {
+int launch_x_L(bfa_env* e, hipStream_t st) {
     constexpr bool DEV = std::is_same<T, float>::value;
     const int R = e->R;
     int logR = 0;
@@ -279,51 +301,52 @@ int launch_x_L(bfa_env* e) {
     const size_t lds = lds_x(e, R, L);
     dim3 grid((unsigned)((rows + R - 1) / R), (unsigned)e->ns);
     auto fn = bfa::k_sweep_x<T, L, DEV>;
-    hipLaunchKernelGGL(fn, grid, dim3(thr), lds, e->stream, sweep_args<T>(e, 0), R, logR);
+    hipLaunchKernelGGL(fn, grid, dim3(thr), lds, st, sweep_args<T>(e, 0), R, logR);
     LAUNCH_CHECK(e, "x sweep");
     return BFA_OK;
 }
 
 template <typename T, int L>
-int launch_xw_L(bfa_env* e) {
+int launch_xw_L(bfa_env* e, hipStream_t st, const Win& w) {
     constexpr bool DEV = std::is_same<T, float>::value;
     const int nx = e->nx;
     const int P = (nx + L - 1) / L;
     const int RW = 64 / P;
-    const long long rows = (long long)e->ny * e->nz;
-    const long long groups = (rows + RW - 1) / RW;
-    const long long want = std::max<long long>(1, 2048 / e->ns);
+    const long long groups = (w.count + RW - 1) / RW;
+    const long long want = std::max<long long>(1, 2048 / w.nd);
     const unsigned bx = (unsigned)std::max<long long>(1, std::min<long long>((groups + 3) / 4, want));
     const bool vec = ((size_t)nx * sizeof(T)) % 16 == 0 && ((size_t)L * sizeof(T)) % 16 == 0;
-    dim3 grid(bx, (unsigned)e->ns);
+    bfa::SweepArgs<T> a = sweep_args<T>(e, 0);
+    a.d0 = w.d0;
+    a.first = w.first;
+    a.count = w.count;
+    dim3 grid(bx, (unsigned)w.nd);
     if (vec)
-        hipLaunchKernelGGL((bfa::k_sweep_x_wave<T, L, DEV, true>), grid, dim3(256), 0, e->stream,
-                           sweep_args<T>(e, 0));
+        hipLaunchKernelGGL((bfa::k_sweep_x_wave<T, L, DEV, true>), grid, dim3(256), 0, st, a);
     else
-        hipLaunchKernelGGL((bfa::k_sweep_x_wave<T, L, DEV, false>), grid, dim3(256), 0, e->stream,
-                           sweep_args<T>(e, 0));
+        hipLaunchKernelGGL((bfa::k_sweep_x_wave<T, L, DEV, false>), grid, dim3(256), 0, st, a);
     LAUNCH_CHECK(e, "x sweep (wave)");
     return BFA_OK;
 }
 
 template <typename T, int L, int AXIS>
-int launch_col_L(bfa_env* e) {
+int launch_col_L(bfa_env* e, hipStream_t st, const Win& w) {
     constexpr bool DEV = std::is_same<T, float>::value;
     const int n = axis_len(e, AXIS);
     const int P = (n + L - 1) / L;
     const int TX = e->TX[AXIS];
-    const long long ncols = AXIS == 1 ? (long long)e->nz * e->nx : (long long)e->ny * e->nx;
     const size_t lds = lds_col(e, n, L, TX);
-    dim3 grid((unsigned)((ncols + TX - 1) / TX), (unsigned)e->ns);
+    bfa::SweepArgs<T> a = sweep_args<T>(e, AXIS);
+    a.d0 = w.d0;
+    a.first = w.first;
+    a.count = w.count;
+    dim3 grid((unsigned)((w.count + TX - 1) / TX), (unsigned)w.nd);
     if (TX == 64)
-        hipLaunchKernelGGL((bfa::k_sweep_col<T, L, AXIS, DEV, 64>), grid, dim3(64 * P), lds,
-                           e->stream, sweep_args<T>(e, AXIS));
+        hipLaunchKernelGGL((bfa::k_sweep_col<T, L, AXIS, DEV, 64>), grid, dim3(64 * P), lds, st, a);
     else if (TX == 32)
-        hipLaunchKernelGGL((bfa::k_sweep_col<T, L, AXIS, DEV, 32>), grid, dim3(32 * P), lds,
-                           e->stream, sweep_args<T>(e, AXIS));
+        hipLaunchKernelGGL((bfa::k_sweep_col<T, L, AXIS, DEV, 32>), grid, dim3(32 * P), lds, st, a);
     else
-        hipLaunchKernelGGL((bfa::k_sweep_col<T, L, AXIS, DEV, 16>), grid, dim3(16 * P), lds,
-                           e->stream, sweep_args<T>(e, AXIS));
+        hipLaunchKernelGGL((bfa::k_sweep_col<T, L, AXIS, DEV, 16>), grid, dim3(16 * P), lds, st, a);
     LAUNCH_CHECK(e, AXIS == 1 ? "y sweep" : "z sweep");
     return BFA_OK;
 }
@@ -358,45 +381,79 @@ int raise_lds_limits(bfa_env* e) {
 }
 
 template <typename T>
-int launch_sweep(bfa_env* e, int axis) {
+int launch_sweep(bfa_env* e, int axis, hipStream_t st, const Win& w) {
     const int L = e->L[axis];
     if (axis == 0 && e->R == 0) {
         switch (L) {
-            case 4: return launch_xw_L<T, 4>(e);
-            case 8: return launch_xw_L<T, 8>(e);
-            case 16: return launch_xw_L<T, 16>(e);
-            case 32: return launch_xw_L<T, 32>(e);
+            case 4: return launch_xw_L<T, 4>(e, st, w);
+            case 8: return launch_xw_L<T, 8>(e, st, w);
+            case 16: return launch_xw_L<T, 16>(e, st, w);
+            case 32: return launch_xw_L<T, 32>(e, st, w);
         }
     } else if (axis == 0) {
         switch (L) {
-            case 4: return launch_x_L<T, 4>(e);
-            case 8: return launch_x_L<T, 8>(e);
-            case 16: return launch_x_L<T, 16>(e);
-            case 32: return launch_x_L<T, 32>(e);
-            case 64: return launch_x_L<T, 64>(e);
+            case 4: return launch_x_L<T, 4>(e, st);
+            case 8: return launch_x_L<T, 8>(e, st);
+            case 16: return launch_x_L<T, 16>(e, st);
+            case 32: return launch_x_L<T, 32>(e, st);
+            case 64: return launch_x_L<T, 64>(e, st);
         }
     } else if (axis == 1) {
         switch (L) {
-            case 4: return launch_col_L<T, 4, 1>(e);
-            case 8: return launch_col_L<T, 8, 1>(e);
-            case 16: return launch_col_L<T, 16, 1>(e);
-            case 32: return launch_col_L<T, 32, 1>(e);
-            case 64: return launch_col_L<T, 64, 1>(e);
+            case 4: return launch_col_L<T, 4, 1>(e, st, w);
+            case 8: return launch_col_L<T, 8, 1>(e, st, w);
+            case 16: return launch_col_L<T, 16, 1>(e, st, w);
+            case 32: return launch_col_L<T, 32, 1>(e, st, w);
+            case 64: return launch_col_L<T, 64, 1>(e, st, w);
         }
     } else {
         switch (L) {
-            case 4: return launch_col_L<T, 4, 2>(e);
-            case 8: return launch_col_L<T, 8, 2>(e);
-            case 16: return launch_col_L<T, 16, 2>(e);
-            case 32: return launch_col_L<T, 32, 2>(e);
-            case 64: return launch_col_L<T, 64, 2>(e);
+            case 4: return launch_col_L<T, 4, 2>(e, st, w);
+            case 8: return launch_col_L<T, 8, 2>(e, st, w);
+            case 16: return launch_col_L<T, 16, 2>(e, st, w);
+            case 32: return launch_col_L<T, 32, 2>(e, st, w);
+            case 64: return launch_col_L<T, 64, 2>(e, st, w);
         }
     }
     return fail(e, BFA_ERR_ARG, "no kernel for chunk length %d", L);
 }
 
 int launch_sweep_any(bfa_env* e, int axis) {
-    return e->prec == BFA_FP32 ? launch_sweep<float>(e, axis) : launch_sweep<double>(e, axis);
+    const Win w = full_win(e, axis);
+    return e->prec == BFA_FP32 ? launch_sweep<float>(e, axis, e->stream, w)
+                               : launch_sweep<double>(e, axis, e->stream, w);
+}
+
+int launch_sweep_win(bfa_env* e, int axis, hipStream_t st, const Win& w) {
+    return e->prec == BFA_FP32 ? launch_sweep<float>(e, axis, st, w)
+                               : launch_sweep<double>(e, axis, st, w);
+}
+
+// x and y sweeps in k-slabs of e->slab planes, per substrate, alternating
+// between two side streams so the x sweep of one slab overlaps the y sweep of
+// another; the y sweep of a slab re-reads what its x sweep just wrote while
+// it is still cache-resident.  Forks from and joins back to e->stream.
+int enqueue_xy_slabs(bfa_env* e) {
+    HIP_TRY(e, hipEventRecord(e->ev_fork, e->stream));
+    HIP_TRY(e, hipStreamWaitEvent(e->aux[0], e->ev_fork, 0));
+    HIP_TRY(e, hipStreamWaitEvent(e->aux[1], e->ev_fork, 0));
+    int q = 0;
+    for (int d = 0; d < e->ns; ++d)
+        for (int k0 = 0; k0 < e->nz; k0 += e->slab, ++q) {
+            const int k1 = std::min(e->nz, k0 + e->slab);
+            hipStream_t st = e->aux[q & 1];
+            const Win wx{d, 1, (long long)k0 * e->ny, (long long)(k1 - k0) * e->ny};
+            const Win wy{d, 1, (long long)k0 * e->nx, (long long)(k1 - k0) * e->nx};
+            int rc = launch_sweep_win(e, 0, st, wx);
+            if (rc) return rc;
+            rc = launch_sweep_win(e, 1, st, wy);
+            if (rc) return rc;
+        }
+    for (int i = 0; i < 2; ++i) {
+        HIP_TRY(e, hipEventRecord(e->ev_join[i], e->aux[i]));
+        HIP_TRY(e, hipStreamWaitEvent(e->stream, e->ev_join[i], 0));
+    }
+    return BFA_OK;
 }
 
 int launch_agents(bfa_env* e) {
@@ -481,20 +538,23 @@ int ensure_precompute(bfa_env* e, double dt) {
     return BFA_OK;
 }
 
-int enqueue_step(bfa_env* e) {
-    for (int a = 0; a < 3; ++a) {
-        int rc = launch_sweep_any(e, a);
-        if (rc) return rc;
-    }
-    return launch_agents(e);
-}
-
 int enqueue_diffusion(bfa_env* e) {
+    if (e->slab > 0 && e->R == 0) {
+        int rc = enqueue_xy_slabs(e);
+        if (rc) return rc;
+        return launch_sweep_any(e, 2);
+    }
     for (int a = 0; a < 3; ++a) {
         int rc = launch_sweep_any(e, a);
         if (rc) return rc;
     }
     return BFA_OK;
+}
+
+int enqueue_step(bfa_env* e) {
+    int rc = enqueue_diffusion(e);
+    if (rc) return rc;
+    return launch_agents(e);
 }
 
 int build_graph(bfa_env* e, double dt) {
@@ -624,6 +684,19 @@ int bfa_create(bfa_env** out, int nx, int ny, int nz, int ns, double dx, int pre
         fail(e, BFA_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(st));
         return bail(BFA_ERR_HIP);
     }
+    for (int i = 0; i < 2; ++i) {
+        st = hipStreamCreateWithFlags(&e->aux[i], hipStreamNonBlocking);
+        if (st == hipSuccess) st = hipEventCreateWithFlags(&e->ev_join[i], hipEventDisableTiming);
+        if (st != hipSuccess) {
+            fail(e, BFA_ERR_HIP, "stream/event creation failed: %s", hipGetErrorString(st));
+            return bail(BFA_ERR_HIP);
+        }
+    }
+    st = hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming);
+    if (st != hipSuccess) {
+        fail(e, BFA_ERR_HIP, "event creation failed: %s", hipGetErrorString(st));
+        return bail(BFA_ERR_HIP);
+    }
     int rc = dalloc(e, &e->u, (size_t)e->N * ns * e->tsize);
     if (rc) return bail(rc);
     st = hipMemsetAsync(e->u, 0, (size_t)e->N * ns * e->tsize, e->stream);
@@ -677,6 +750,12 @@ void bfa_destroy(bfa_env* e) {
     dfree(e->agA);
     dfree(e->agB);
     dfree(e->agC);
+    for (int i = 0; i < 2; ++i) {
+        if (e->aux[i]) (void)hipStreamSynchronize(e->aux[i]);
+        if (e->aux[i]) (void)hipStreamDestroy(e->aux[i]);
+        if (e->ev_join[i]) (void)hipEventDestroy(e->ev_join[i]);
+    }
+    if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -1032,14 +1111,23 @@ int bfa_time_kernels(bfa_env* e, double dt, int reps, double* ms) {
     if (!e || !ms || reps < 1) return fail(e, BFA_ERR_ARG, "bad argument");
     int rc = ready(e, dt);
     if (rc) return rc;
+    const bool slabs = e->slab > 0 && e->R == 0;
     hipEvent_t ev[6];
     for (auto& x : ev) HIP_TRY(e, hipEventCreate(&x));
     double acc[5] = {0, 0, 0, 0, 0};
     for (int r = 0; r < reps && rc == BFA_OK; ++r) {
         (void)hipEventRecord(ev[0], e->stream);
-        for (int a = 0; a < 3 && rc == BFA_OK; ++a) {
-            rc = launch_sweep_any(e, a);
-            (void)hipEventRecord(ev[a + 1], e->stream);
+        if (slabs) {
+            rc = enqueue_xy_slabs(e);
+            (void)hipEventRecord(ev[1], e->stream);
+            (void)hipEventRecord(ev[2], e->stream);
+            if (rc == BFA_OK) rc = launch_sweep_any(e, 2);
+            (void)hipEventRecord(ev[3], e->stream);
+        } else {
+            for (int a = 0; a < 3 && rc == BFA_OK; ++a) {
+                rc = launch_sweep_any(e, a);
+                (void)hipEventRecord(ev[a + 1], e->stream);
+            }
         }
         if (rc == BFA_OK) rc = launch_agents(e);
         (void)hipEventRecord(ev[4], e->stream);
@@ -1057,6 +1145,18 @@ int bfa_time_kernels(bfa_env* e, double dt, int reps, double* ms) {
     for (int k = 0; k < 5; ++k) ms[k] = acc[k] / reps;
     return BFA_OK;
 }
+
+int bfa_set_slab(bfa_env* e, int planes) {
+    if (!e) return BFA_ERR_ARG;
+    if (planes < 0) return fail(e, BFA_ERR_ARG, "slab planes must be >= 0, got %d", planes);
+    if (planes > 0 && e->R != 0)
+        return fail(e, BFA_ERR_ARG, "slab schedule needs the wave-local x sweep (rows_x == 0)");
+    e->slab = std::min(planes, e->nz);
+    invalidate_graph(e);
+    return BFA_OK;
+}
+
+int bfa_get_slab(const bfa_env* e) { return e ? e->slab : -1; }
 
 int bfa_copy_kernel(size_t bytes, int reps, double* ms) {
     if (!ms || reps < 1 || bytes < 16) return BFA_ERR_ARG;
@@ -1111,6 +1211,7 @@ int bfa_set_tiles(bfa_env* e, int Lx, int Ly, int Lz, int rows_x, int cols_yz) {
         e->TXreq = sTXreq;
         return rc;
     }
+    if (e->R != 0) e->slab = 0;
     invalidate_graph(e);
     return BFA_OK;
 }

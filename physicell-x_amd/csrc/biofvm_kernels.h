@@ -60,6 +60,12 @@ struct SweepArgs {
     long long N;      // nx*ny*nz
     int nx, ny, nz, ns;
     unsigned reg;     // bit f set: face f registered
+    // Launch window: substrates d0 + blockIdx.y; lines [first, first + count)
+    // of each substrate (x sweep: rows r = k*ny + j; y sweep: columns
+    // c = k*nx + i; z sweep: columns c = j*nx + i).  A k-slab [k0, k1) is the
+    // window first = k0*ny (x) or k0*nx (y), count = (k1-k0)*ny or (k1-k0)*nx.
+    int d0;
+    long long first, count;
 };
 
 // ---- chunk recurrences (positions s..s+L-1 of a padded line) ----------------
@@ -322,11 +328,12 @@ __global__ __launch_bounds__(256) void k_sweep_x_wave(SweepArgs<T> a) {
     const int rw = lane / P;
     const int p = lane - rw * P;
     const bool lane_on = rw < RW;
-    const int d = blockIdx.y;
+    const int d = a.d0 + blockIdx.y;
     const int s = p * L;
     const int valid = lane_on ? (nx - s < L ? nx - s : L) : 0;
-    const int rows = ny * nz;
-    const int ngroups = (rows + RW - 1) / RW;
+    const int row0 = (int)a.first;
+    const int rows = row0 + (int)a.count;
+    const int ngroups = ((int)a.count + RW - 1) / RW;
     const int wstride = gridDim.x * (blockDim.x >> 6);
     const T k = a.k[d];
     const T bt = DEV ? a.beta[d] : T(0);
@@ -341,7 +348,7 @@ __global__ __launch_bounds__(256) void k_sweep_x_wave(SweepArgs<T> a) {
     }
     T* ub = a.u + (long long)d * a.N;
     for (int g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); g < ngroups; g += wstride) {
-        const int r = g * RW + rw;
+        const int r = row0 + g * RW + rw;
         const bool have = lane_on && r < rows;
         T cur[L];
         if (have)
@@ -444,12 +451,11 @@ __global__ __launch_bounds__(1024) void k_sweep_col(SweepArgs<T> a) {
     const int t = threadIdx.x;
     const int lane = t % TX;
     const int p = t / TX;
-    const int d = blockIdx.y;
+    const int d = a.d0 + blockIdx.y;
     stage_coefs<T>(s_inv, s_cp, a.inv + (long long)d * n, a.cp + (long long)d * n, n, P * L, t,
                    blockDim.x);
-    const long long ncols = AXIS == 1 ? (long long)nz * nx : (long long)ny * nx;
-    const long long c = (long long)blockIdx.x * TX + lane;
-    const bool act = c < ncols;
+    const long long c = a.first + (long long)blockIdx.x * TX + lane;
+    const bool act = c < a.first + a.count;
     long long base = 0, stride;
     int ci = 0, co = 0;  // i and the other transverse coordinate (k for y, j for z)
     if (act) {
