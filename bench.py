@@ -84,7 +84,8 @@ def make_workload(n, ns, n_agents, seed):
 def make_env(pkg, w, n, ns, precision, tiles=None):
     env = pkg.Microenvironment(n, n, n, ns, 20.0, precision)
     if tiles is not None:
-        env.set_tiles(tiles["Lx"], tiles["Ly"], tiles["Lz"], tiles["rows_x"], tiles["cols_yz"])
+        env.set_tiles(tiles["Lx"], tiles["Ly"], tiles["Lz"], tiles["rows_x"], tiles["cols_y"],
+                      tiles["cols_z"])
         env.set_slab(tiles.get("slab", 0))
     env.set_coefficients(w["D"], w["decay"])
     env.set_dirichlet(np.ones(ns, np.uint8), np.ones(6, np.uint8), np.tile(w["dval"], (6, 1)))
@@ -238,7 +239,8 @@ def main():
         "copy_kernel_GBps": copy_gbs,
         "graph_ms_per_step": 1e3 * gwall32 / args.steps,
         "tiles": dict(Lx=tiles32[0], Ly=tiles32[1], Lz=tiles32[2], rows_x=tiles32[3],
-                      cols_yz=tiles32[4], slab=slab32, autotuned=not args.no_tune),
+                      cols_y=tiles32[4], cols_z=tiles32[5], slab=slab32,
+                      autotuned=not args.no_tune),
     }
     out.update(res)
     if not args.no_cpu:

@@ -137,12 +137,11 @@ int bfa_copy_kernel(size_t bytes, int reps, double *ms);
  * in-register line solve per axis (4, 8, 16, 32 or 64; 0 = automatic); the
  * x-sweep form: rows_x == 0 selects the wave-local x sweep (a row's chunks
  * live in one wavefront), rows_x in {1,2,4,...,64} the LDS-tiled x sweep
- * with that many rows per tile; cols_yz, the x-columns per y/z sweep block
- * (16, 32 or 64; 0 = automatic).  On error the previous shape is kept.
- * bfa_get_tiles writes {Lx, Ly, Lz, rows_x, cols_yz} (cols_yz = -1 when the
- * y and z sweeps use different widths). */
-int bfa_set_tiles(bfa_env *env, int Lx, int Ly, int Lz, int rows_x, int cols_yz);
-int bfa_get_tiles(const bfa_env *env, int *out5);
+ * with that many rows per tile; cols_y / cols_z, the x-columns per y / z
+ * sweep block (16, 32 or 64; 0 = automatic).  On error the previous shape is
+ * kept.  bfa_get_tiles writes {Lx, Ly, Lz, rows_x, cols_y, cols_z}. */
+int bfa_set_tiles(bfa_env *env, int Lx, int Ly, int Lz, int rows_x, int cols_y, int cols_z);
+int bfa_get_tiles(const bfa_env *env, int *out6);
 
 /* Sweep schedule.  planes == 0: each sweep covers the whole volume.
  * planes > 0: the x and y sweeps run in slabs of `planes` k-planes per

@@ -80,7 +80,7 @@ def lib():
     L.bfa_synchronize.argtypes = [_vp]
     L.bfa_time_kernels.argtypes = [_vp, ctypes.c_double, ctypes.c_int, _dp]
     L.bfa_copy_kernel.argtypes = [ctypes.c_size_t, ctypes.c_int, _dp]
-    L.bfa_set_tiles.argtypes = [_vp] + [ctypes.c_int] * 5
+    L.bfa_set_tiles.argtypes = [_vp] + [ctypes.c_int] * 6
     L.bfa_get_tiles.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
     L.bfa_set_slab.argtypes = [_vp, ctypes.c_int]
     L.bfa_get_slab.argtypes = [_vp]
@@ -199,14 +199,14 @@ class Microenvironment:
         self._check(lib().bfa_get_fp32_basis(self._h, _d(b), _d(s)), "get_fp32_basis")
         return b, s
 
-    def set_tiles(self, Lx=0, Ly=0, Lz=0, rows_x=0, cols_yz=0):
+    def set_tiles(self, Lx=0, Ly=0, Lz=0, rows_x=0, cols_y=0, cols_z=0):
         """Kernel shapes (see bfa_set_tiles); 0 means automatic."""
         self._check(lib().bfa_set_tiles(self._h, int(Lx), int(Ly), int(Lz), int(rows_x),
-                                        int(cols_yz)), "set_tiles")
+                                        int(cols_y), int(cols_z)), "set_tiles")
 
     def tiles(self):
-        """(Lx, Ly, Lz, rows_x, cols_yz) currently in use."""
-        v = (ctypes.c_int * 5)()
+        """(Lx, Ly, Lz, rows_x, cols_y, cols_z) currently in use."""
+        v = (ctypes.c_int * 6)()
         self._check(lib().bfa_get_tiles(self._h, v), "get_tiles")
         return tuple(v)
 

@@ -8,8 +8,7 @@ sweep kernels on the GPU (HIP events around each launch, via
   (a row's ceil(nx/L) <= 64 chunks share one wavefront), or the LDS-tiled
   form with R rows per tile and chunk length L;
 * y and z sweeps: TX in {16, 32, 64} x-columns per block and chunk length L,
-  with TX * ceil(n/L) <= 1024 threads.  TX is shared by y and z; L is chosen
-  per axis;
+  with TX * ceil(n/L) <= 1024 threads, chosen per axis;
 * schedule: whole-volume sweeps, or x/y sweeps in k-slabs of a few planes
   (chosen by whole-step time).
 
@@ -47,7 +46,7 @@ def _time(env, dt, reps, shape):
 
 def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=3, cache_path=None,
              verbose=False):
-    """Return {"Lx", "Ly", "Lz", "rows_x", "cols_yz", "ms": {...}, "trials": [...]}."""
+    """Return {"Lx", "rows_x", "Ly", "cols_y", "Lz", "cols_z", "slab", "ms", "trials"}."""
     key = _key(nx, ny, nz, ns, precision)
     if key in _cache:
         return _cache[key]
@@ -68,34 +67,29 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=3, cache_path=
         env.upload(20.0 + rng.standard_normal(nx * ny * nz * ns), "soa")
         best_x = None
         for L in X_WAVE_L:
-            ms = _time(env, dt, reps, (L, 0, 0, 0, 0))
+            ms = _time(env, dt, reps, (L, 0, 0, 0, 0, 0))
             if ms is not None:
                 trials.append({"axis": "x", "form": "wave", "L": L, "ms": ms[0]})
                 if best_x is None or ms[0] < best_x[0]:
                     best_x = (ms[0], L, 0)
         for R, L in X_LDS:
-            ms = _time(env, dt, reps, (L, 0, 0, R, 0))
+            ms = _time(env, dt, reps, (L, 0, 0, R, 0, 0))
             if ms is not None:
                 trials.append({"axis": "x", "form": "lds", "R": R, "L": L, "ms": ms[0]})
                 if best_x is None or ms[0] < best_x[0]:
                     best_x = (ms[0], L, R)
-        best_col = None
+        best_y = best_z = None
         for TX in COL_TX:
-            by = bz = None
             for L in COL_L:
-                ms = _time(env, dt, reps, (best_x[1], L, L, best_x[2], TX))
+                ms = _time(env, dt, reps, (best_x[1], L, L, best_x[2], TX, TX))
                 if ms is None:
                     continue
                 trials.append({"axis": "yz", "TX": TX, "L": L, "ms_y": ms[1], "ms_z": ms[2]})
-                if by is None or ms[1] < by[0]:
-                    by = (ms[1], L)
-                if bz is None or ms[2] < bz[0]:
-                    bz = (ms[2], L)
-            if by is not None and bz is not None:
-                tot = by[0] + bz[0]
-                if best_col is None or tot < best_col[0]:
-                    best_col = (tot, TX, by, bz)
-        shape = (best_x[1], best_col[2][1], best_col[3][1], best_x[2], best_col[1])
+                if best_y is None or ms[1] < best_y[0]:
+                    best_y = (ms[1], L, TX)
+                if best_z is None or ms[2] < best_z[0]:
+                    best_z = (ms[2], L, TX)
+        shape = (best_x[1], best_y[1], best_z[1], best_x[2], best_y[2], best_z[2])
         env.set_tiles(*shape)
         env.time_kernels(dt, 1)
         step0 = env.time_kernels(dt, reps)[4]
@@ -114,10 +108,9 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=3, cache_path=
                 if t < best_slab[0]:
                     best_slab = (t, planes)
             env.set_slab(0)
-        res = {"Lx": best_x[1], "rows_x": best_x[2], "cols_yz": best_col[1],
-               "Ly": best_col[2][1], "Lz": best_col[3][1], "slab": best_slab[1],
-               "ms": {"x": best_x[0], "y": best_col[2][0], "z": best_col[3][0],
-                      "step": best_slab[0]},
+        res = {"Lx": best_x[1], "rows_x": best_x[2], "Ly": best_y[1], "cols_y": best_y[2],
+               "Lz": best_z[1], "cols_z": best_z[2], "slab": best_slab[1],
+               "ms": {"x": best_x[0], "y": best_y[0], "z": best_z[0], "step": best_slab[0]},
                "trials": trials}
     finally:
         env.close()
@@ -139,5 +132,5 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=3, cache_path=
 
 
 def apply(env, res):
-    env.set_tiles(res["Lx"], res["Ly"], res["Lz"], res["rows_x"], res["cols_yz"])
+    env.set_tiles(res["Lx"], res["Ly"], res["Lz"], res["rows_x"], res["cols_y"], res["cols_z"])
     env.set_slab(res.get("slab", 0))

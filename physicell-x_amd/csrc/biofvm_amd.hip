@@ -81,7 +81,7 @@ struct bfa_env {
 
     int Lreq[3] = {0, 0, 0};
     int Rreq = 0;
-    int TXreq = 0;
+    int TXreq[3] = {0, 0, 0};
     int L[3] = {0, 0, 0};
     int R = 0;
     int TX[3] = {0, 64, 64};
@@ -166,7 +166,7 @@ int choose_tiles(bfa_env* e) {
     // rows with TX*P <= 1024 threads.
     for (int a = 1; a < 3; ++a) {
         const int n = axis_len(e, a);
-        int TX = e->TXreq, L = e->Lreq[a];
+        int TX = e->TXreq[a], L = e->Lreq[a];
         if (TX == 0 && L == 0) {
             // keep the register array at 16 nodes (no spills, full occupancy)
             // and narrow the block when the line needs more chunks
@@ -313,7 +313,7 @@ int launch_xw_L(bfa_env* e, hipStream_t st, const Win& w) {
     const int P = (nx + L - 1) / L;
     const int RW = 64 / P;
     const long long groups = (w.count + RW - 1) / RW;
-    const long long want = std::max<long long>(1, 2048 / w.nd);
+    const long long want = std::max<long long>(1, 1024 / w.nd);
     const unsigned bx = (unsigned)std::max<long long>(1, std::min<long long>((groups + 3) / 4, want));
     const bool vec = ((size_t)nx * sizeof(T)) % 16 == 0 && ((size_t)L * sizeof(T)) % 16 == 0;
     bfa::SweepArgs<T> a = sweep_args<T>(e, 0);
@@ -1189,26 +1189,28 @@ int bfa_copy_kernel(size_t bytes, int reps, double* ms) {
     return BFA_OK;
 }
 
-int bfa_set_tiles(bfa_env* e, int Lx, int Ly, int Lz, int rows_x, int cols_yz) {
+int bfa_set_tiles(bfa_env* e, int Lx, int Ly, int Lz, int rows_x, int cols_y, int cols_z) {
     if (!e) return BFA_ERR_ARG;
-    int sLreq[3], sL[3], sTX[3];
+    int sLreq[3], sL[3], sTX[3], sTXreq[3];
     std::copy(e->Lreq, e->Lreq + 3, sLreq);
     std::copy(e->L, e->L + 3, sL);
     std::copy(e->TX, e->TX + 3, sTX);
-    const int sRreq = e->Rreq, sR = e->R, sTXreq = e->TXreq;
+    std::copy(e->TXreq, e->TXreq + 3, sTXreq);
+    const int sRreq = e->Rreq, sR = e->R;
     e->Lreq[0] = Lx;
     e->Lreq[1] = Ly;
     e->Lreq[2] = Lz;
     e->Rreq = rows_x;
-    e->TXreq = cols_yz;
+    e->TXreq[1] = cols_y;
+    e->TXreq[2] = cols_z;
     int rc = apply_tiles(e);
     if (rc) {
         std::copy(sLreq, sLreq + 3, e->Lreq);
         std::copy(sL, sL + 3, e->L);
         std::copy(sTX, sTX + 3, e->TX);
+        std::copy(sTXreq, sTXreq + 3, e->TXreq);
         e->Rreq = sRreq;
         e->R = sR;
-        e->TXreq = sTXreq;
         return rc;
     }
     if (e->R != 0) e->slab = 0;
@@ -1216,13 +1218,14 @@ int bfa_set_tiles(bfa_env* e, int Lx, int Ly, int Lz, int rows_x, int cols_yz) {
     return BFA_OK;
 }
 
-int bfa_get_tiles(const bfa_env* e, int* out5) {
-    if (!e || !out5) return BFA_ERR_ARG;
-    out5[0] = e->L[0];
-    out5[1] = e->L[1];
-    out5[2] = e->L[2];
-    out5[3] = e->R;
-    out5[4] = e->TX[1] == e->TX[2] ? e->TX[1] : -1;
+int bfa_get_tiles(const bfa_env* e, int* out6) {
+    if (!e || !out6) return BFA_ERR_ARG;
+    out6[0] = e->L[0];
+    out6[1] = e->L[1];
+    out6[2] = e->L[2];
+    out6[3] = e->R;
+    out6[4] = e->TX[1];
+    out6[5] = e->TX[2];
     return BFA_OK;
 }
 

@@ -314,6 +314,24 @@ __device__ __forceinline__ void store_chunk(const T (&x)[L], T* __restrict__ dst
     }
 }
 
+// Overwrite the Dirichlet nodes of the chunk [s, s+L) of a row of n nodes:
+// the ends take v_lo / v_hi, and when the whole row is on a face (rowb) the
+// interior takes v_in.  Static register indices only.
+template <typename T, int L>
+__device__ __forceinline__ void dirichlet_row(T (&x)[L], int s, int n, bool rowb, T v_lo,
+                                              T v_hi, T v_in) {
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+        const int pos = s + m;
+        if (pos < n) {
+            T v = rowb ? v_in : x[m];
+            v = pos == 0 ? v_lo : v;
+            v = pos == n - 1 ? v_hi : v;
+            x[m] = v;
+        }
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ T shfl_up_t(T v, int d) { return __shfl_up(v, d, 64); }
 template <typename T>
@@ -357,20 +375,20 @@ __global__ __launch_bounds__(256) void k_sweep_x_wave(SweepArgs<T> a) {
 #pragma unroll
             for (int m = 0; m < L; ++m) cur[m] = T(0);
         }
-        int jj = 0, kk = 0;
+        // Dirichlet values of this row: both ends, and the interior when the
+        // whole row lies on a y or z face.
+        T v_lo = T(0), v_hi = T(0), v_in = T(0);
         bool rowb = false;
-        if (have) {
-            kk = r / ny;
-            jj = r - kk * ny;
+        if (have && dact) {
+            const int kk = r / ny;
+            const int jj = r - kk * ny;
             rowb = (jj == 0 || jj == ny - 1 || kk == 0 || kk == nz - 1);
-            if (dact && (rowb || s == 0 || s + valid == nx)) {
-                for (int m = 0; m < valid; ++m) {
-                    const int pos = s + m;
-                    if (rowb || pos == 0 || pos == nx - 1)
-                        cur[m] = a.dval[dir_slot(pos, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
-                }
-            }
+            v_lo = a.dval[dir_slot(0, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
+            v_hi = a.dval[dir_slot(nx - 1, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
+            v_in = a.dval[dir_slot(-1, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
         }
+        const bool dir_on = have && dact;
+        if (dir_on) dirichlet_row<T, L>(cur, s, nx, rowb, v_lo, v_hi, v_in);
         if (DEV) {
 #pragma unroll
             for (int m = 0; m < L; ++m) cur[m] = (m < valid) ? cur[m] - bt : T(0);
@@ -421,13 +439,7 @@ __global__ __launch_bounds__(256) void k_sweep_x_wave(SweepArgs<T> a) {
             cur[m] = xn;
         }
         if (have) {
-            if (dact && (rowb || s == 0 || s + valid == nx)) {
-                for (int m = 0; m < valid; ++m) {
-                    const int pos = s + m;
-                    if (rowb || pos == 0 || pos == nx - 1)
-                        cur[m] = a.dval[dir_slot(pos, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
-                }
-            }
+            if (dir_on) dirichlet_row<T, L>(cur, s, nx, rowb, v_lo, v_hi, v_in);
             store_chunk<T, L, VEC>(cur, ub + r * nx + s, valid);
         }
     }
@@ -484,15 +496,14 @@ __global__ __launch_bounds__(1024) void k_sweep_col(SweepArgs<T> a) {
     if (act && dact) {
         const bool lineb = AXIS == 1 ? (ci == 0 || ci == nx - 1 || co == 0 || co == nz - 1)
                                      : (ci == 0 || ci == nx - 1 || co == 0 || co == ny - 1);
-#pragma unroll
-        for (int m = 0; m < L; ++m) {
-            const int pos = s + m;
-            if (pos < n && (lineb || pos == 0 || pos == n - 1)) {
-                const int slot = AXIS == 1 ? dir_slot(ci, pos, co, nx, ny, nz, a.reg)
-                                           : dir_slot(ci, co, pos, nx, ny, nz, a.reg);
-                x[m] = a.dval[slot * a.ns + d];
-            }
-        }
+        const int s_lo = AXIS == 1 ? dir_slot(ci, 0, co, nx, ny, nz, a.reg)
+                                   : dir_slot(ci, co, 0, nx, ny, nz, a.reg);
+        const int s_hi = AXIS == 1 ? dir_slot(ci, n - 1, co, nx, ny, nz, a.reg)
+                                   : dir_slot(ci, co, n - 1, nx, ny, nz, a.reg);
+        const int s_in = AXIS == 1 ? dir_slot(ci, -1, co, nx, ny, nz, a.reg)
+                                   : dir_slot(ci, co, -1, nx, ny, nz, a.reg);
+        dirichlet_row<T, L>(x, s, n, lineb, a.dval[s_lo * a.ns + d], a.dval[s_hi * a.ns + d],
+                            a.dval[s_in * a.ns + d]);
     }
     if (act) {
 #pragma unroll

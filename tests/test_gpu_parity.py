@@ -231,8 +231,9 @@ def test_changing_dt_recomputes_coefficients(pkg):
         env.close()
 
 
-TILES = [(4, 4, 4, 1, 64), (8, 16, 32, 8, 32), (32, 8, 16, 32, 16), (16, 32, 4, 64, 64),
-         (4, 8, 8, 0, 16), (8, 4, 16, 0, 32), (16, 16, 4, 0, 64), (32, 32, 32, 0, 32)]
+TILES = [(4, 4, 4, 1, 64, 64), (8, 16, 32, 8, 32, 16), (32, 8, 16, 32, 16, 64),
+         (16, 32, 4, 64, 64, 32), (4, 8, 8, 0, 16, 16), (8, 4, 16, 0, 32, 64),
+         (16, 16, 4, 0, 64, 16), (32, 32, 32, 0, 32, 32)]
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
@@ -260,8 +261,8 @@ def test_x_sweep_forms_at_odd_row_lengths(pkg, nx):
     orc = oracle_for(p)
     u = p.u0.copy()
     orc.diffusion_step(u, p.dt)
-    for shape in [(0, 0, 0, 0, 0), (4, 0, 0, 0, 0), (8, 0, 0, 0, 0), (16, 0, 0, 0, 0),
-                  (32, 0, 0, 0, 0), (0, 0, 0, 8, 0)]:
+    for shape in [(0, 0, 0, 0, 0, 0), (4, 0, 0, 0, 0, 0), (8, 0, 0, 0, 0, 0), (16, 0, 0, 0, 0, 0),
+                  (32, 0, 0, 0, 0, 0), (0, 0, 0, 8, 0, 0)]:
         env = gpu_env(pkg, p, "fp64")
         try:
             try:
@@ -306,7 +307,8 @@ def test_invalid_tiles_are_rejected_and_state_kept(pkg):
     env = gpu_env(pkg, p, "fp64")
     try:
         before = env.tiles()
-        for bad in [(3, 0, 0, 0, 0), (0, 0, 0, 3, 0), (0, 0, 0, 0, 48), (64, 0, 0, 0, 0)]:
+        for bad in [(3, 0, 0, 0, 0, 0), (0, 0, 0, 3, 0, 0), (0, 0, 0, 0, 48, 0), (64, 0, 0, 0, 0, 0),
+                    (0, 0, 0, 0, 0, 8)]:
             with pytest.raises(pkg.BioFVMError):
                 env.set_tiles(*bad)
             assert env.tiles() == before
