@@ -153,6 +153,53 @@ int bfa_get_tiles(const bfa_env *env, int *out6);
 int bfa_set_slab(bfa_env *env, int planes);
 int bfa_get_slab(const bfa_env *env);
 
+/* ---- batches of independent microenvironments -------------------------------
+ *
+ * A batch advances many microenvironments of different shapes and substrate
+ * counts together: one persistent kernel per sweep covers every member, and a
+ * call of bfa_batch_step replays one captured HIP graph for all its steps.
+ * The reference has no counterpart (each Microenvironment is stepped on its
+ * own through simulate_diffusion_decay); per member the arithmetic is that of
+ * bfa_diffusion_decay + bfa_secretion_uptake.  Members can be added and
+ * removed between calls.  One precision per batch.  Member edges: x <= 512,
+ * y and z <= 1024.
+ */
+typedef struct bfa_batch bfa_batch;
+
+int bfa_batch_create(bfa_batch **out, int precision);
+void bfa_batch_destroy(bfa_batch *batch);
+const char *bfa_batch_last_error(const bfa_batch *batch);
+/* Device bytes one member of this shape occupies (for admission control). */
+size_t bfa_batch_member_bytes(int nx, int ny, int nz, int ns, int precision);
+/* Add a member: grid, dx, coefficients and Dirichlet state as for bfa_create,
+ * bfa_set_coefficients and bfa_set_dirichlet.  Writes the member id. */
+int bfa_batch_add(bfa_batch *batch, int nx, int ny, int nz, int ns, double dx, const double *D,
+                  const double *decay, const uint8_t *active, const uint8_t *face_registered,
+                  const double *face_values, int *member_id);
+/* Free a member's device memory; its id may be reused by a later add. */
+int bfa_batch_remove(bfa_batch *batch, int member_id);
+int bfa_batch_upload(bfa_batch *batch, int member_id, const double *field, int layout);
+int bfa_batch_download(bfa_batch *batch, int member_id, double *field, int layout);
+int bfa_batch_set_agents(bfa_batch *batch, int member_id, int n, const int32_t *voxel,
+                         const uint8_t *active, const double *c1, const double *c2,
+                         const double *e2);
+/* Set every voxel of substrate d to values[d] (initial conditions without a
+ * host transfer). */
+int bfa_batch_fill(bfa_batch *batch, int member_id, const double *values);
+/* How many of each following bfa_batch_step call's steps this member takes
+ * (it stays untouched for the rest of the call); default: all. */
+int bfa_batch_set_member_steps(bfa_batch *batch, int member_id, int steps);
+/* nsteps x (diffusion + agents) for every member, asynchronous on the
+ * batch's stream. */
+int bfa_batch_step(bfa_batch *batch, double dt, int nsteps);
+int bfa_batch_synchronize(bfa_batch *batch);
+int bfa_batch_total_mass(bfa_batch *batch, int member_id, double *out);
+/* Kernel shapes: x chunk length (8, 16, 32), y / z chunk length (4, 8, 16)
+ * and columns per block (16, 32, 64); 0 = automatic.  get writes
+ * {Lx, Ly, Lz, cols_y, cols_z}. */
+int bfa_batch_set_tiles(bfa_batch *batch, int Lx, int Ly, int Lz, int cols_y, int cols_z);
+int bfa_batch_get_tiles(const bfa_batch *batch, int *out5);
+
 #ifdef __cplusplus
 }
 #endif

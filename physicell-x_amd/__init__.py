@@ -84,6 +84,27 @@ def lib():
     L.bfa_get_tiles.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
     L.bfa_set_slab.argtypes = [_vp, ctypes.c_int]
     L.bfa_get_slab.argtypes = [_vp]
+    L.bfa_batch_create.argtypes = [P, ctypes.c_int]
+    L.bfa_batch_destroy.argtypes = [_vp]
+    L.bfa_batch_destroy.restype = None
+    L.bfa_batch_last_error.argtypes = [_vp]
+    L.bfa_batch_last_error.restype = ctypes.c_char_p
+    L.bfa_batch_member_bytes.argtypes = [ctypes.c_int] * 5
+    L.bfa_batch_member_bytes.restype = ctypes.c_size_t
+    L.bfa_batch_add.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_double, _dp, _dp, _u8, _u8, _dp,
+                                ctypes.POINTER(ctypes.c_int)]
+    L.bfa_batch_remove.argtypes = [_vp, ctypes.c_int]
+    L.bfa_batch_upload.argtypes = [_vp, ctypes.c_int, _dp, ctypes.c_int]
+    L.bfa_batch_download.argtypes = [_vp, ctypes.c_int, _dp, ctypes.c_int]
+    L.bfa_batch_set_agents.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _i32, _u8, _dp, _dp, _dp]
+    L.bfa_batch_set_member_steps.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+    L.bfa_batch_fill.argtypes = [_vp, ctypes.c_int, _dp]
+    L.bfa_batch_step.argtypes = [_vp, ctypes.c_double, ctypes.c_int]
+    L.bfa_batch_synchronize.argtypes = [_vp]
+    L.bfa_batch_total_mass.argtypes = [_vp, ctypes.c_int, _dp]
+    L.bfa_batch_set_tiles.argtypes = [_vp] + [ctypes.c_int] * 5
+    L.bfa_batch_get_tiles.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
     _lib = L
     return L
 
@@ -267,3 +288,126 @@ class Microenvironment:
         ms = np.zeros(5)
         self._check(lib().bfa_time_kernels(self._h, float(dt), int(reps), _d(ms)), "time_kernels")
         return ms
+
+
+def member_bytes(nx, ny, nz, ns, precision="fp32"):
+    """Device bytes a batch member of this shape occupies."""
+    return int(lib().bfa_batch_member_bytes(int(nx), int(ny), int(nz), int(ns), _PREC[precision]))
+
+
+class Batch:
+    """Many independent microenvironments advanced together (one persistent
+    kernel per sweep over all members, one HIP graph per run of steps).
+
+    Members are added with the same arguments as a ``Microenvironment`` plus
+    its coefficients and Dirichlet state; each member's result equals what a
+    ``Microenvironment`` of the same state computes.
+    """
+
+    def __init__(self, precision="fp32"):
+        self._h = None
+        L = lib()
+        h = ctypes.c_void_p()
+        rc = L.bfa_batch_create(ctypes.byref(h), _PREC[precision])
+        if rc:
+            raise BioFVMError(f"bfa_batch_create: {L.bfa_last_error(None).decode()}")
+        self._h = h
+        self.precision = _PREC[precision]
+        self.members = {}
+
+    def close(self):
+        if self._h is not None:
+            lib().bfa_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc:
+            msg = lib().bfa_batch_last_error(self._h).decode()
+            raise BioFVMError(f"{what}: status {rc}: {msg}")
+
+    def add(self, nx, ny, nz, ns, dx, D, decay, active=None, face_registered=None,
+            face_values=None):
+        """Add a member; returns its id."""
+        if active is None:
+            active = np.zeros(ns, np.uint8)
+        if face_registered is None:
+            face_registered = np.zeros(6, np.uint8)
+        if face_values is None:
+            face_values = np.ones((6, ns))
+        act = np.ascontiguousarray(active, np.uint8).reshape(ns)
+        reg = np.ascontiguousarray(face_registered, np.uint8).reshape(6)
+        fv = _f64(face_values, 6 * ns)
+        mid = ctypes.c_int()
+        self._check(lib().bfa_batch_add(self._h, int(nx), int(ny), int(nz), int(ns), float(dx),
+                                        _d(_f64(D, ns)), _d(_f64(decay, ns)),
+                                        act.ctypes.data_as(_u8), reg.ctypes.data_as(_u8), _d(fv),
+                                        ctypes.byref(mid)), "batch_add")
+        self.members[mid.value] = (int(nx), int(ny), int(nz), int(ns))
+        return mid.value
+
+    def remove(self, mid):
+        self._check(lib().bfa_batch_remove(self._h, int(mid)), "batch_remove")
+        self.members.pop(mid, None)
+
+    def upload(self, mid, field, layout="ref"):
+        nx, ny, nz, ns = self.members[mid]
+        f = _f64(field, nx * ny * nz * ns)
+        self._check(lib().bfa_batch_upload(self._h, int(mid), _d(f), _LAYOUT[layout]), "batch_upload")
+
+    def download(self, mid, layout="ref"):
+        nx, ny, nz, ns = self.members[mid]
+        out = np.empty(nx * ny * nz * ns)
+        self._check(lib().bfa_batch_download(self._h, int(mid), _d(out), _LAYOUT[layout]),
+                    "batch_download")
+        return out
+
+    def set_agents(self, mid, voxel, c1, c2, e2, active=None):
+        ns = self.members[mid][3]
+        voxel = np.ascontiguousarray(voxel, np.int32)
+        n = voxel.size
+        act = None if active is None else np.ascontiguousarray(active, np.uint8)
+        self._check(lib().bfa_batch_set_agents(self._h, int(mid), n, voxel.ctypes.data_as(_i32),
+                                               None if act is None else act.ctypes.data_as(_u8),
+                                               _d(_f64(c1, n * ns)), _d(_f64(c2, n * ns)),
+                                               _d(_f64(e2, n * ns))), "batch_set_agents")
+
+    def fill(self, mid, values):
+        """Set every voxel of substrate d to values[d]."""
+        ns = self.members[mid][3]
+        self._check(lib().bfa_batch_fill(self._h, int(mid), _d(_f64(values, ns))), "batch_fill")
+
+    def set_member_steps(self, mid, steps):
+        self._check(lib().bfa_batch_set_member_steps(self._h, int(mid), int(steps)),
+                    "batch_set_member_steps")
+
+    def step(self, dt, nsteps=1):
+        self._check(lib().bfa_batch_step(self._h, float(dt), int(nsteps)), "batch_step")
+
+    def synchronize(self):
+        self._check(lib().bfa_batch_synchronize(self._h), "batch_synchronize")
+
+    def total_mass(self, mid):
+        out = np.zeros(self.members[mid][3])
+        self._check(lib().bfa_batch_total_mass(self._h, int(mid), _d(out)), "batch_total_mass")
+        return out
+
+    def set_tiles(self, Lx=0, Ly=0, Lz=0, cols_y=0, cols_z=0):
+        self._check(lib().bfa_batch_set_tiles(self._h, int(Lx), int(Ly), int(Lz), int(cols_y),
+                                              int(cols_z)), "batch_set_tiles")
+
+    def tiles(self):
+        v = (ctypes.c_int * 5)()
+        self._check(lib().bfa_batch_get_tiles(self._h, v), "batch_get_tiles")
+        return tuple(v)

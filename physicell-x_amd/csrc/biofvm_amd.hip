@@ -18,6 +18,7 @@
 
 #include "biofvm_amd.h"
 #include "biofvm_aux_kernels.h"
+#include "biofvm_batch_kernels.h"
 #include "biofvm_kernels.h"
 
 namespace {
@@ -1230,3 +1231,6 @@ int bfa_get_tiles(const bfa_env* e, int* out6) {
 }
 
 }  // extern "C"
+
+// ---- batches of independent microenvironments ------------------------------
+#include "biofvm_batch_host.inc"

@@ -283,6 +283,17 @@ __global__ void k_soa_to_f64(const T* __restrict__ in, double* __restrict__ out,
         out[(long long)d * N + v] = fma((double)in[(long long)d * N + v], ss, bb);
 }
 
+// Constant fill per substrate: out = (value[d] - b[d]) * rs[d].
+template <typename T>
+__global__ void k_fill_const(T* __restrict__ out, long long N, const double* __restrict__ value,
+                             const double* __restrict__ b, const double* __restrict__ rs) {
+    const int d = blockIdx.y;
+    const T v = (T)((value[d] - b[d]) * rs[d]);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += (long long)gridDim.x * blockDim.x)
+        out[(long long)d * N + i] = v;
+}
+
 // Plain 16-byte copy, used only to calibrate HBM byte counters.
 __global__ void k_copy16(const uint4* __restrict__ in, uint4* __restrict__ out, long long n) {
     for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n;

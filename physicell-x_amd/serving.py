@@ -1,0 +1,218 @@
+"""Serving loop: continuous batching of diffusion-decay solve jobs on one GPU.
+
+A job is one microenvironment (grid, substrates, coefficients, Dirichlet
+state, initial condition, agents) to be advanced a given number of steps.  The
+server keeps every admitted job's field resident in device memory between
+steps and advances all of them together through one ``Batch``: each
+iteration runs up to ``steps_per_iteration`` steps, every job taking exactly
+the steps it still needs (the batch skips a member's work once its count for
+the call is used).  Jobs that arrive while others are on the GPU wait in a
+FIFO queue and are admitted at the next iteration boundary, as soon as the
+device-memory budget has room for them; finished jobs are retired at once
+(result downloaded, memory freed), which admits the next ones.
+
+The kernel shapes are fixed when the server starts, so a job's result does
+not depend on which other jobs shared its iterations.
+
+There is no counterpart in the reference (it advances one Microenvironment
+per process); per job the arithmetic is that of
+``Microenvironment.simulate_diffusion_decay`` + ``simulate_cell_sources_and_sinks``.
+"""
+from __future__ import annotations
+
+import collections
+import time
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+
+@dataclass
+class Job:
+    nx: int
+    ny: int
+    nz: int
+    ns: int
+    steps: int
+    D: np.ndarray
+    decay: np.ndarray
+    dx: float = 20.0
+    dt: float = 0.01
+    active: Optional[np.ndarray] = None
+    face_registered: Optional[np.ndarray] = None
+    face_values: Optional[np.ndarray] = None
+    init_values: Optional[np.ndarray] = None   # uniform start per substrate ...
+    init_field: Optional[np.ndarray] = None    # ... or a full field (reference layout)
+    agent_voxel: Optional[np.ndarray] = None
+    agent_c1: Optional[np.ndarray] = None
+    agent_c2: Optional[np.ndarray] = None
+    agent_e2: Optional[np.ndarray] = None
+    want_field: bool = False
+    # filled in by the server
+    id: int = -1
+    arrival: float = 0.0
+    admitted: float = 0.0
+    finished: float = 0.0
+    remaining: int = 0
+    member: int = -1
+    bytes: int = 0
+    mass: Optional[np.ndarray] = None
+    result: Optional[np.ndarray] = None
+    error: Optional[str] = None
+
+    @property
+    def updates(self):
+        return self.nx * self.ny * self.nz * self.ns * self.steps
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+class Server:
+    """Continuous-batching solve server over one ``Batch``.
+
+    ``hbm_budget`` bounds the device bytes of admitted jobs (per
+    ``member_bytes``); a job larger than the whole budget is rejected at
+    submission.  ``max_edge`` fixes the kernel shapes for the largest grid
+    edge the server accepts.
+    """
+
+    def __init__(self, pkg, precision="fp32", hbm_budget=200e9, steps_per_iteration=10,
+                 max_edge=256, dt=0.01, clock=time.perf_counter):
+        self.pkg = pkg
+        self.precision = precision
+        self.batch = pkg.Batch(precision)
+        self.budget = float(hbm_budget)
+        self.k = int(steps_per_iteration)
+        self.max_edge = int(max_edge)
+        self.dt = float(dt)
+        self.clock = clock
+        # chunk lengths that cover max_edge: x rows in <= 64 lanes, y/z lines
+        # in <= 32 chunk rows of 32 columns
+        lx = 8 if max_edge <= 512 else 16
+        lyz = 8 if max_edge <= 256 else 16
+        self.batch.set_tiles(lx, lyz, lyz, 32, 32)
+        self.queue = collections.deque()
+        self.active = {}
+        self.done = []
+        self.rejected = []
+        self.used = 0
+        self.iterations = 0
+        self._next_id = 0
+
+    def close(self):
+        self.batch.close()
+
+    # -- submission and admission ------------------------------------------------
+    def submit(self, job: Job, arrival: Optional[float] = None) -> int:
+        job.id = self._next_id
+        self._next_id += 1
+        job.arrival = self.clock() if arrival is None else arrival
+        job.remaining = int(job.steps)
+        job.bytes = self.pkg.member_bytes(job.nx, job.ny, job.nz, job.ns, self.precision)
+        if max(job.nx, job.ny, job.nz) > self.max_edge:
+            job.error = f"edge above the server limit {self.max_edge}"
+            self.rejected.append(job)
+        elif job.bytes > self.budget:
+            job.error = "larger than the device-memory budget"
+            self.rejected.append(job)
+        elif abs(job.dt - self.dt) > 0.0:
+            job.error = f"dt {job.dt} differs from the server's {self.dt}"
+            self.rejected.append(job)
+        else:
+            self.queue.append(job)
+        return job.id
+
+    def _admit(self):
+        admitted = 0
+        while self.queue and self.used + self.queue[0].bytes <= self.budget:
+            job = self.queue.popleft()
+            ns = job.ns
+            act = job.active if job.active is not None else np.zeros(ns, np.uint8)
+            reg = job.face_registered if job.face_registered is not None else np.zeros(6, np.uint8)
+            fv = job.face_values if job.face_values is not None else np.ones((6, ns))
+            try:
+                mid = self.batch.add(job.nx, job.ny, job.nz, ns, job.dx, job.D, job.decay, act, reg, fv)
+            except self.pkg.BioFVMError as exc:  # device allocation failed: retry later
+                self.queue.appendleft(job)
+                if not self.active:
+                    raise
+                del exc
+                break
+            job.member = mid
+            if job.init_field is not None:
+                self.batch.upload(mid, job.init_field)
+            else:
+                vals = job.init_values if job.init_values is not None else np.zeros(ns)
+                self.batch.fill(mid, vals)
+            if job.agent_voxel is not None and len(job.agent_voxel):
+                self.batch.set_agents(mid, job.agent_voxel, job.agent_c1, job.agent_c2, job.agent_e2)
+            job.admitted = self.clock()
+            self.used += job.bytes
+            self.active[mid] = job
+            admitted += 1
+        return admitted
+
+    # -- the loop ------------------------------------------------------------------------
+    def busy(self):
+        return bool(self.queue or self.active)
+
+    def iterate(self):
+        """Admit what fits, run one iteration of up to k steps, retire finished jobs.
+        Returns the jobs finished in this iteration."""
+        self._admit()
+        if not self.active:
+            return []
+        k = min(self.k, max(j.remaining for j in self.active.values()))
+        for mid, job in self.active.items():
+            self.batch.set_member_steps(mid, min(k, job.remaining))
+        self.batch.step(self.dt, k)
+        self.batch.synchronize()
+        self.iterations += 1
+        finished = []
+        for mid in list(self.active):
+            job = self.active[mid]
+            job.remaining -= min(k, job.remaining)
+            if job.remaining == 0:
+                if job.want_field:
+                    job.result = self.batch.download(mid)
+                job.mass = self.batch.total_mass(mid) if job.want_field else None
+                self.batch.remove(mid)
+                self.used -= job.bytes
+                job.finished = self.clock()
+                del self.active[mid]
+                self.done.append(job)
+                finished.append(job)
+        return finished
+
+    def drain(self):
+        while self.busy():
+            self.iterate()
+
+
+def run_arrivals(server: Server, jobs, offsets, clock=time.perf_counter):
+    """Submit jobs at the given arrival offsets (seconds from now) while the
+    server iterates; returns (wall seconds, finished jobs)."""
+    order = np.argsort(offsets, kind="stable")
+    pending = collections.deque((offsets[i], jobs[i]) for i in order)
+    t0 = clock()
+    while pending or server.busy():
+        now = clock() - t0
+        while pending and pending[0][0] <= now:
+            off, job = pending.popleft()
+            server.submit(job, arrival=t0 + off)
+        if server.busy():
+            server.iterate()
+        elif pending:
+            time.sleep(max(0.0, min(0.01, pending[0][0] - (clock() - t0))))
+    return clock() - t0, list(server.done)
+
+
+def latency_stats(jobs):
+    lat = np.array([j.latency for j in jobs])
+    if lat.size == 0:
+        return {"p50_s": None, "p99_s": None, "mean_s": None}
+    return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "max_s": float(lat.max())}

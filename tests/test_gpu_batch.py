@@ -1,0 +1,199 @@
+"""GPU tests of the batched form: many independent microenvironments of
+different shapes advanced by one persistent kernel per sweep.
+
+Bars: every member against the CPU oracle at the plain-solve bars (FP64
+<= 1e-12, FP32 <= 1e-5 relative max-norm per substrate); a member's result is
+bit-identical alone and in the mix (the per-line arithmetic does not depend on
+the rest of the batch).
+"""
+import numpy as np
+import pytest
+
+from helpers import agents_for, load_pkg, make_problem, oracle_for, relmax
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": 1e-12, "fp32": 1e-5}
+
+
+@pytest.fixture(scope="module")
+def pkg():
+    return load_pkg()
+
+
+SHAPES = [(32, 40, 48, 1), (97, 33, 64, 3), (160, 160, 32, 2), (45, 130, 77, 4), (64, 64, 64, 2),
+          (33, 159, 41, 1), (120, 36, 150, 3), (50, 50, 50, 4)]
+
+
+def problems(seed0=100, dirichlet=("all", "mixed", "off")):
+    out = []
+    for i, shape in enumerate(SHAPES):
+        nvox = shape[0] * shape[1] * shape[2]
+        p = make_problem(*shape, seed=seed0 + i, n_agents=max(10, nvox // 1000),
+                         dirichlet=dirichlet[i % len(dirichlet)], export=5.0, collide=3)
+        out.append(p)
+    return out
+
+
+def add(batch, p):
+    mid = batch.add(p.nx, p.ny, p.nz, p.ns, p.dx, p.D, p.decay, p.active, p.reg, p.fv)
+    batch.upload(mid, p.u0)
+    if p.voxel.size:
+        batch.set_agents(mid, p.voxel, p.c1, p.c2, p.e2, p.agent_active)
+    return mid
+
+
+def oracle_run(p, steps):
+    orc = oracle_for(p)
+    ag = agents_for(p) if p.voxel.size else None
+    u = p.u0.copy()
+    for _ in range(steps):
+        orc.diffusion_step(u, p.dt)
+        if ag is not None:
+            orc.secretion_uptake(u, ag)
+    return u
+
+
+def per_substrate_err(g, u, ns):
+    gg, uu = g.reshape(-1, ns), u.reshape(-1, ns)
+    return max(relmax(gg[:, d], uu[:, d]) for d in range(ns))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_members_match_oracle(pkg, precision):
+    ps = problems()
+    steps = 4
+    with pkg.Batch(precision) as b:
+        ids = [add(b, p) for p in ps]
+        b.step(ps[0].dt, steps)
+        b.synchronize()
+        for p, mid in zip(ps, ids):
+            err = per_substrate_err(b.download(mid), oracle_run(p, steps), p.ns)
+            assert err <= TOL[precision], (p.nx, p.ny, p.nz, p.ns, err)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_member_alone_equals_member_in_mix(pkg, precision):
+    ps = problems(seed0=200)
+    steps = 3
+    with pkg.Batch(precision) as mix:
+        ids = [add(mix, p) for p in ps]
+        mix.step(ps[0].dt, steps)
+        for k in (1, 4, 6):
+            with pkg.Batch(precision) as alone:
+                aid = add(alone, ps[k])
+                alone.set_tiles(*mix.tiles())  # same kernel shapes
+                alone.step(ps[k].dt, steps)
+                assert np.array_equal(alone.download(aid), mix.download(ids[k]))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_member_equals_plain_solve(pkg, precision):
+    p = problems(seed0=300)[3]
+    steps = 3
+    env = pkg.Microenvironment(p.nx, p.ny, p.nz, p.ns, p.dx, precision)
+    try:
+        env.set_coefficients(p.D, p.decay)
+        env.set_dirichlet(p.active, p.reg, p.fv)
+        env.upload(p.u0)
+        env.set_agents(p.voxel, p.c1, p.c2, p.e2, p.agent_active)
+        env.step(p.dt, steps)
+        plain = env.download()
+    finally:
+        env.close()
+    with pkg.Batch(precision) as b:
+        mid = add(b, p)
+        b.step(p.dt, steps)
+        got = b.download(mid)
+    assert per_substrate_err(got, plain, p.ns) <= (1e-13 if precision == "fp64" else 2e-6)
+
+
+def test_member_steps_and_membership_changes(pkg):
+    ps = problems(seed0=400)[:5]
+    with pkg.Batch("fp64") as b:
+        ids = [add(b, p) for p in ps]
+        b.set_member_steps(ids[1], 2)
+        b.set_member_steps(ids[3], 0)
+        b.step(ps[0].dt, 5)
+        for k, want in ((0, 5), (1, 2), (2, 5), (3, 0), (4, 5)):
+            err = relmax(b.download(ids[k]), oracle_run(ps[k], want))
+            assert err <= 1e-12, (k, want, err)
+        # remove a member, add a new one, step again: survivors continue exactly
+        b.remove(ids[2])
+        q = problems(seed0=500)[6]
+        nid = add(b, q)
+        for m in ids:
+            if m != ids[2]:
+                b.set_member_steps(m, 3)
+        b.step(ps[0].dt, 3)
+        assert relmax(b.download(nid), oracle_run(q, 3)) <= 1e-12
+        assert relmax(b.download(ids[0]), oracle_run(ps[0], 8)) <= 1e-12
+        assert relmax(b.download(ids[1]), oracle_run(ps[1], 5)) <= 1e-12
+        assert relmax(b.download(ids[3]), oracle_run(ps[3], 3)) <= 1e-12
+
+
+def test_batch_total_mass_and_tiles(pkg):
+    ps = problems(seed0=600)[:4]
+    with pkg.Batch("fp64") as b:
+        ids = [add(b, p) for p in ps]
+        for k, p in enumerate(ps):
+            assert np.allclose(b.total_mass(ids[k]), oracle_for(p).total_mass(p.u0), rtol=1e-13, atol=0)
+        b.step(ps[0].dt, 1)
+        t = b.tiles()
+        assert t[0] in (8, 16, 32) and t[3] in (16, 32, 64)
+        with pytest.raises(pkg.BioFVMError):
+            b.set_tiles(7, 0, 0, 0, 0)
+        assert b.tiles() == t
+        for shape in [(16, 4, 8, 64, 16), (32, 16, 16, 16, 32)]:
+            b.set_tiles(*shape)
+            b.step(ps[0].dt, 1)
+        for k, p in enumerate(ps):
+            assert relmax(b.download(ids[k]), oracle_run(p, 3)) <= 1e-12
+
+
+def served_jobs(S, seed0):
+    shapes = [(40, 52, 36, 2, 7), (64, 64, 64, 1, 12), (33, 90, 47, 3, 5), (100, 40, 60, 2, 9),
+              (48, 48, 48, 4, 3), (70, 35, 80, 1, 15)]
+    jobs, probs = [], []
+    for i, (nx, ny, nz, ns, steps) in enumerate(shapes):
+        p = make_problem(nx, ny, nz, ns, seed=seed0 + i, n_agents=nx * ny * nz // 2000 + 5,
+                         dirichlet=("all", "mixed", "off")[i % 3], export=3.0, collide=2)
+        j = S.Job(nx=nx, ny=ny, nz=nz, ns=ns, steps=steps, D=p.D, decay=p.decay, dx=p.dx, dt=p.dt,
+                  active=p.active, face_registered=p.reg, face_values=p.fv, init_field=p.u0,
+                  agent_voxel=p.voxel[p.agent_active == 1], agent_c1=p.c1[p.agent_active == 1],
+                  agent_c2=p.c2[p.agent_active == 1], agent_e2=p.e2[p.agent_active == 1],
+                  want_field=True)
+        jobs.append(j)
+        probs.append(p)
+    return jobs, probs
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_served_jobs_equal_alone_and_oracle(pkg, precision):
+    import importlib
+    S = importlib.import_module("physicellx_amd.serving")
+    jobs, probs = served_jobs(S, 700)
+    # a budget that holds about three of these jobs at once forces queueing
+    budget = 3 * max(pkg.member_bytes(j.nx, j.ny, j.nz, j.ns, precision) for j in jobs)
+    srv = S.Server(pkg, precision, hbm_budget=budget, steps_per_iteration=4, max_edge=128)
+    try:
+        # staggered arrivals: half now, the rest while the first ones run
+        for j in jobs[:3]:
+            srv.submit(j)
+        srv.iterate()
+        for j in jobs[3:]:
+            srv.submit(j)
+        srv.drain()
+        tiles = srv.batch.tiles()
+    finally:
+        srv.close()
+    assert len(srv.done) == len(jobs) and not srv.rejected
+    assert max(j.admitted for j in jobs[3:]) > min(j.admitted for j in jobs[:3])
+    for j, p in zip(jobs, probs):
+        with pkg.Batch(precision) as alone:
+            alone.set_tiles(tiles[0], tiles[1], tiles[2], tiles[3], tiles[4])
+            mid = add(alone, p)
+            alone.step(p.dt, j.steps)
+            assert np.array_equal(alone.download(mid), j.result), (j.nx, j.ny, j.nz, j.ns)
+        err = per_substrate_err(j.result, oracle_run(p, j.steps), p.ns)
+        assert err <= TOL[precision], (j.nx, j.ny, j.nz, j.ns, err)
