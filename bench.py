@@ -149,6 +149,227 @@ def cpu_baseline(w, n, ns, budget_s=12.0, max_steps=40):
                        f"{steps} steps after the Thomas precompute, {el:.1f} s, {threads} threads")
 
 
+def ensemble_members(n_env, seed):
+    """256-member ensemble: edges i.i.d. U{32..160}, 1-4 substrates, D
+    log-uniform [1e3, 1e5], decay U[0.01, 1], Dirichlet U[0, 40] on all faces,
+    start at the Dirichlet value, agents = 0.1% of voxels at uniform voxels
+    with secretion/uptake rates U[0, 10]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_env):
+        nx, ny, nz = (int(v) for v in rng.integers(32, 161, 3))
+        ns = int(rng.integers(1, 5))
+        D = 10 ** rng.uniform(3, 5, ns)
+        decay = rng.uniform(0.01, 1.0, ns)
+        dval = rng.uniform(0, 40, ns)
+        nvox = nx * ny * nz
+        na = max(1, nvox // 1000)
+        voxel = rng.integers(0, nvox, na).astype(np.int32)
+        c1, c2, e2 = agent_constants(0.01, 20.0 ** 3, np.full(na, 2494.0), rng.uniform(0, 10, (na, ns)),
+                                     rng.uniform(0, 10, (na, ns)), np.zeros((na, ns)),
+                                     np.full((na, ns), 40.0))
+        out.append(dict(nx=nx, ny=ny, nz=nz, ns=ns, D=D, decay=decay, dval=dval, voxel=voxel,
+                        c1=c1, c2=c2, e2=e2))
+    return out
+
+
+def run_ensemble(pkg, n_env=256, steps=100, repeats=3, seed=4, sequential=True):
+    """Batched ensemble vs sequential plain solves of the same members."""
+    mem = ensemble_members(n_env, seed)
+    updates = sum(m["nx"] * m["ny"] * m["nz"] * m["ns"] for m in mem) * steps
+    res = {"members": n_env, "steps": steps, "updates_per_run": updates}
+    fields = {}
+    for prec in ("fp32", "fp64"):
+        b = pkg.Batch(prec)
+        ids = []
+        for m in mem:
+            mid = b.add(m["nx"], m["ny"], m["nz"], m["ns"], 20.0, m["D"], m["decay"],
+                        np.ones(m["ns"], np.uint8), np.ones(6, np.uint8), np.tile(m["dval"], (6, 1)))
+            b.fill(mid, m["dval"])
+            b.set_agents(mid, m["voxel"], m["c1"], m["c2"], m["e2"])
+            ids.append(mid)
+        b.step(0.01, steps)  # warm-up: capture and first replay
+        b.synchronize()
+        lat = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            b.step(0.01, steps)
+            b.synchronize()
+            lat.append(time.perf_counter() - t0)
+        lat = np.array(lat)
+        res[prec] = {"value": updates / float(np.median(lat)), "p50_s": float(np.percentile(lat, 50)),
+                     "p99_s": float(np.percentile(lat, 99)), "runs": repeats, "tiles": list(b.tiles())}
+        fields[prec] = [b.download(ids[k], "soa") for k in (0, n_env // 2, n_env - 1)]
+        b.close()
+    errs = [float(np.max(np.abs(a - c)) / np.max(np.abs(c))) for a, c in zip(fields["fp32"], fields["fp64"])]
+    res["fp32_vs_fp64_max_rel_err"] = max(errs)
+    if sequential:
+        envs = []
+        for m in mem:
+            e = pkg.Microenvironment(m["nx"], m["ny"], m["nz"], m["ns"], 20.0, "fp32")
+            e.set_coefficients(m["D"], m["decay"])
+            e.set_dirichlet(np.ones(m["ns"], np.uint8), np.ones(6, np.uint8), np.tile(m["dval"], (6, 1)))
+            e.upload(np.tile(m["dval"], m["nx"] * m["ny"] * m["nz"]))
+            e.set_agents(m["voxel"], m["c1"], m["c2"], m["e2"])
+            e.step(0.01, 1)
+            e.synchronize()
+            envs.append(e)
+        t0 = time.perf_counter()
+        for e in envs:
+            e.step(0.01, steps)
+            e.synchronize()
+        seq = time.perf_counter() - t0
+        for e in envs:
+            e.close()
+        res["sequential_fp32"] = {"value": updates / seq, "seconds": seq}
+        res["speedup_vs_sequential_fp32"] = res["fp32"]["value"] / (updates / seq)
+    return res
+
+
+def serving_requests(n_req, seed, rate=200.0, edge=(48, 256), ns_max=8, steps=(10, 1000)):
+    """Poisson arrivals; edges U{48..256}, substrates U{1..8}, steps U{10..1000},
+    D log-uniform [1e3, 1e5], decay U[0.01, 1], Dirichlet U[0, 40], agents =
+    0.05% of voxels with random rates."""
+    rng = np.random.default_rng(seed)
+    offs = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    reqs = []
+    for _ in range(n_req):
+        nx, ny, nz = (int(v) for v in rng.integers(edge[0], edge[1] + 1, 3))
+        ns = int(rng.integers(1, ns_max + 1))
+        st = int(rng.integers(steps[0], steps[1] + 1))
+        D = 10 ** rng.uniform(3, 5, ns)
+        decay = rng.uniform(0.01, 1.0, ns)
+        dval = rng.uniform(0, 40, ns)
+        nvox = nx * ny * nz
+        na = max(1, nvox // 2000)
+        voxel = rng.integers(0, nvox, na).astype(np.int32)
+        c1, c2, e2 = agent_constants(0.01, 20.0 ** 3, np.full(na, 2494.0), rng.uniform(0, 10, (na, ns)),
+                                     rng.uniform(0, 10, (na, ns)), np.zeros((na, ns)),
+                                     np.full((na, ns), 40.0))
+        reqs.append(dict(nx=nx, ny=ny, nz=nz, ns=ns, steps=st, D=D, decay=decay, dval=dval,
+                         voxel=voxel, c1=c1, c2=c2, e2=e2))
+    return reqs, offs
+
+
+def run_serving(pkg, n_req=40, seed=5, rate=200.0, budget=200e9, k=10, spot=2):
+    import importlib
+    S = importlib.import_module("physicellx_amd.serving")
+    reqs, offs = serving_requests(n_req, seed, rate)
+
+    def job(r, want):
+        return S.Job(nx=r["nx"], ny=r["ny"], nz=r["nz"], ns=r["ns"], steps=r["steps"], D=r["D"],
+                     decay=r["decay"], active=np.ones(r["ns"], np.uint8),
+                     face_registered=np.ones(6, np.uint8), face_values=np.tile(r["dval"], (6, 1)),
+                     init_values=r["dval"], agent_voxel=r["voxel"], agent_c1=r["c1"],
+                     agent_c2=r["c2"], agent_e2=r["e2"], want_field=want)
+
+    sizes = [r["nx"] * r["ny"] * r["nz"] * r["ns"] for r in reqs]
+    spot_ids = list(np.argsort(sizes)[:spot])
+    jobs = [job(r, i in spot_ids) for i, r in enumerate(reqs)]
+    srv = S.Server(pkg, "fp32", hbm_budget=budget, steps_per_iteration=k, max_edge=256)
+    wall, done = S.run_arrivals(srv, jobs, offs)
+    srv.close()
+    updates = sum(j.updates for j in done)
+    res = {"requests": n_req, "rate_per_s": rate, "steps_per_iteration": k,
+           "offered_updates_per_s": updates / float(offs[-1]) if offs[-1] > 0 else None,
+           "value": updates / wall, "unit": "voxel·substrate updates/s", "wall_s": wall,
+           "iterations": srv.iterations}
+    res.update(S.latency_stats(done))
+    # FP64 spot check of the smallest requests, run alone
+    errs = []
+    for i in spot_ids:
+        r, j = reqs[i], jobs[i]
+        with pkg.Batch("fp64") as b:
+            mid = b.add(r["nx"], r["ny"], r["nz"], r["ns"], 20.0, r["D"], r["decay"],
+                        np.ones(r["ns"], np.uint8), np.ones(6, np.uint8), np.tile(r["dval"], (6, 1)))
+            b.fill(mid, r["dval"])
+            b.set_agents(mid, r["voxel"], r["c1"], r["c2"], r["e2"])
+            b.step(0.01, r["steps"])
+            ref = b.download(mid)
+        errs.append(float(np.max(np.abs(j.result - ref)) / np.max(np.abs(ref))))
+    res["fp64_spot_check_max_rel_err"] = max(errs) if errs else None
+    # sequential plain solves of a sample of the same requests (throughput)
+    sample = [reqs[i] for i in np.random.default_rng(seed + 1).choice(n_req, min(4, n_req), replace=False)]
+    tot_u, tot_t = 0, 0.0
+    for r in sample:
+        e = pkg.Microenvironment(r["nx"], r["ny"], r["nz"], r["ns"], 20.0, "fp32")
+        e.set_coefficients(r["D"], r["decay"])
+        e.set_dirichlet(np.ones(r["ns"], np.uint8), np.ones(6, np.uint8), np.tile(r["dval"], (6, 1)))
+        e.upload(np.tile(r["dval"], r["nx"] * r["ny"] * r["nz"]))
+        e.set_agents(r["voxel"], r["c1"], r["c2"], r["e2"])
+        e.step(0.01, 1)
+        e.synchronize()
+        t0 = time.perf_counter()
+        e.step(0.01, r["steps"])
+        e.synchronize()
+        tot_t += time.perf_counter() - t0
+        tot_u += r["nx"] * r["ny"] * r["nz"] * r["ns"] * r["steps"]
+        e.close()
+    res["sequential_plain_sample"] = {"value": tot_u / tot_t, "requests": len(sample)}
+    res["speedup_vs_sequential"] = res["value"] / (tot_u / tot_t)
+    return res
+
+
+def run_large(pkg, n=800, steps=20, seed=3):
+    """800^3 x 2 FP32 scaled-deviation form vs the FP64 path: an oxygen-like
+    substrate (D=1e5, decay 0.1, Dirichlet 38) and a signal (D=1e3, decay 0.4,
+    Dirichlet 0); 1e6 agents in a radius-6 mm spheroid at the centre, density
+    weighted toward the rim, uptake 10/min on substrate 0 and secretion 1/min
+    on substrate 1 (saturation density 40 on substrate 1)."""
+    rng = np.random.default_rng(seed)
+    ns = 2
+    D = np.array([1e5, 1e3])
+    decay = np.array([0.1, 0.4])
+    dval = np.array([38.0, 0.0])
+    na = 1_000_000
+    # radius r in [0, R] with density ~ exp(-((R - r)/w)^2) r^2 (rim-weighted)
+    R = 6000.0
+    r = R * (1 - np.abs(rng.normal(0, 0.15, na)))
+    r = np.clip(r, 0, R)
+    v = rng.normal(size=(na, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    pos = v * r[:, None]
+    L = n * 20.0
+    idx = np.clip(np.floor((pos + L / 2) / 20.0).astype(np.int64), 0, n - 1)
+    voxel = ((idx[:, 0] * n + idx[:, 1]) * n + idx[:, 2]).astype(np.int32)
+    S = np.zeros((na, ns))
+    U = np.zeros((na, ns))
+    U[:, 0] = 10.0
+    S[:, 1] = 1.0
+    rs = np.zeros((na, ns))
+    rs[:, 1] = 40.0
+    c1, c2, e2 = agent_constants(0.01, 20.0 ** 3, np.full(na, 2494.0), S, U, np.zeros((na, ns)), rs)
+    field = np.empty(ns * n ** 3)
+    field[: n ** 3] = 38.0
+    field[n ** 3:] = 0.0
+    out = {"grid": [n, n, n], "substrates": ns, "agents": na, "steps": steps}
+    envs = {}
+    for prec in ("fp32", "fp64"):
+        e = pkg.Microenvironment(n, n, n, ns, 20.0, prec)
+        e.set_coefficients(D, decay)
+        e.set_dirichlet(np.ones(ns, np.uint8), np.ones(6, np.uint8), np.tile(dval, (6, 1)))
+        e.upload(field, "soa")
+        e.set_agents(voxel, c1, c2, e2)
+        e.step(0.01, 1)
+        e.synchronize()
+        t0 = time.perf_counter()
+        e.step(0.01, steps - 1)
+        e.synchronize()
+        dt_ = time.perf_counter() - t0
+        out[prec] = {"value": n ** 3 * ns * (steps - 1) / dt_, "ms_per_step": 1e3 * dt_ / (steps - 1),
+                     "tiles": list(e.tiles())}
+        envs[prec] = e
+    f32 = envs["fp32"].download("soa").reshape(ns, -1)
+    f64 = envs["fp64"].download("soa").reshape(ns, -1)
+    m32, m64 = envs["fp32"].total_mass(), envs["fp64"].total_mass()
+    out["max_rel_err_per_substrate"] = [float(np.max(np.abs(f32[d] - f64[d])) / np.max(np.abs(f64[d])))
+                                        for d in range(ns)]
+    out["mass_drift_per_substrate"] = [float(abs(m32[d] - m64[d]) / abs(m64[d])) for d in range(ns)]
+    for e in envs.values():
+        e.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,12 +383,23 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--mode", default="default", choices=["default", "single", "ensemble", "serve", "large"])
+    ap.add_argument("--requests", type=int, default=40)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the domain is resident on a single MI355X)", file=sys.stderr)
         sys.exit(2)
     pkg = load_pkg()
+    if args.mode == "ensemble":
+        print(json.dumps({"ensemble": run_ensemble(pkg)}))
+        return
+    if args.mode == "serve":
+        print(json.dumps({"serving": run_serving(pkg, n_req=args.requests)}))
+        return
+    if args.mode == "large":
+        print(json.dumps({"large": run_large(pkg)}))
+        return
     n, ns = args.n, args.ns
     dt = 0.01
     w = make_workload(n, ns, args.agents, args.seed)
@@ -243,6 +475,13 @@ def main():
                       autotuned=not args.no_tune),
     }
     out.update(res)
+    if args.mode == "default":
+        ens = run_ensemble(pkg, repeats=3)
+        srv = run_serving(pkg, n_req=args.requests)
+        out["batched_solve"] = {"p50_s": ens["fp32"]["p50_s"], "p99_s": ens["fp32"]["p99_s"],
+                                "workload": "256-member ensemble, 100 steps per solve, FP32"}
+        out["ensemble"] = ens
+        out["serving"] = srv
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, n, ns)
     print(json.dumps(out))

@@ -168,6 +168,7 @@ __global__ __launch_bounds__(256) void kb_sweep_x(const EnvDesc<T>* __restrict__
         const int nx = a.nx;
         const int P = (nx + L - 1) / L;
         const int RW = 64 / P;
+        if (RW == 0) continue;  // row longer than 64 chunks: the host never lists such items
         const int rw = lane / P;
         const int p = lane - rw * P;
         const bool lane_on = rw < RW;

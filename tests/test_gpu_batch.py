@@ -144,7 +144,7 @@ def test_batch_total_mass_and_tiles(pkg):
         with pytest.raises(pkg.BioFVMError):
             b.set_tiles(7, 0, 0, 0, 0)
         assert b.tiles() == t
-        for shape in [(16, 4, 8, 64, 16), (32, 16, 16, 16, 32)]:
+        for shape in [(16, 16, 8, 64, 16), (32, 8, 16, 16, 32)]:
             b.set_tiles(*shape)
             b.step(ps[0].dt, 1)
         for k, p in enumerate(ps):
