@@ -1,0 +1,331 @@
+// BioFVM_amd.cpp -- C++ host side over the C ABI (see BioFVM_amd.h).
+#include "BioFVM_amd.h"
+
+#include <cmath>
+#include <cstring>
+#include <iostream>
+
+namespace BioFVM_amd {
+
+// ---- mesh --------------------------------------------------------------------------
+
+void Cartesian_Mesh::resize(double x_start, double x_end, double y_start, double y_end,
+                            double z_start, double z_end, double dx_new, double dy_new,
+                            double dz_new) {
+    dx = dx_new;
+    dy = dy_new;
+    dz = dz_new;
+    const double eps = 1e-16;  // BioFVM/BioFVM_mesh.cpp:1467-1473
+    x_nodes = (int)std::ceil(eps + (x_end - x_start) / dx);
+    y_nodes = (int)std::ceil(eps + (y_end - y_start) / dy);
+    z_nodes = (int)std::ceil(eps + (z_end - z_start) / dz);
+    const double tol = 1e-16;
+    uniform_mesh = !(std::fabs(dx - dy) > tol || std::fabs(dy - dz) > tol || std::fabs(dx - dz) > tol);
+    bounding_box[0] = x_start;
+    bounding_box[1] = y_start;
+    bounding_box[2] = z_start;
+    bounding_box[3] = x_end;
+    bounding_box[4] = y_end;
+    bounding_box[5] = z_end;
+    dV = dx * dy * dz;
+}
+
+bool Cartesian_Mesh::is_position_valid(double x, double y, double z) const {
+    return !(x < bounding_box[0] || x > bounding_box[3] || y < bounding_box[1] ||
+             y > bounding_box[4] || z < bounding_box[2] || z > bounding_box[5]);
+}
+
+int Cartesian_Mesh::nearest_voxel_index(const std::vector<double>& position) const {
+    // floor of the offset from the box corner, truncated into the grid
+    // (BioFVM/BioFVM_mesh.cpp:1825-1843)
+    unsigned int i = (unsigned int)std::floor((position[0] - bounding_box[0]) / dx);
+    unsigned int j = (unsigned int)std::floor((position[1] - bounding_box[1]) / dy);
+    unsigned int k = (unsigned int)std::floor((position[2] - bounding_box[2]) / dz);
+    if (i >= (unsigned)x_nodes) i = x_nodes - 1;
+    if (j >= (unsigned)y_nodes) j = y_nodes - 1;
+    if (k >= (unsigned)z_nodes) k = z_nodes - 1;
+    return (int)((i * (unsigned)y_nodes + j) * (unsigned)z_nodes + k);
+}
+
+// ---- agents --------------------------------------------------------------------------
+
+void Basic_Agent::register_microenvironment(Microenvironment* m) {
+    microenvironment = m;
+    const size_t ns = (size_t)m->number_of_densities();
+    secretion_rates.resize(ns, 0.0);
+    uptake_rates.resize(ns, 0.0);
+    saturation_densities.resize(ns, 0.0);
+    net_export_rates.resize(ns, 0.0);
+    volume_is_changed = true;
+}
+
+void Basic_Agent::update_voxel_index() {
+    const Cartesian_Mesh& mesh = microenvironment->mesh;
+    if (!mesh.is_position_valid(position[0], position[1], position[2])) {
+        current_voxel_index = -1;
+        is_active = false;
+        return;
+    }
+    current_voxel_index = mesh.nearest_voxel_index(position);
+}
+
+void Basic_Agent::set_internal_uptake_constants(double dt) {
+    // dp/dt = S (T - p) - U p, implicit in p; the net export is explicit.
+    const double W = microenvironment->mesh.dV;
+    const double f = dt * volume / W;
+    const size_t ns = secretion_rates.size();
+    cell_source_sink_solver_temp1.assign(ns, 0.0);
+    cell_source_sink_solver_temp2.assign(ns, 1.0);
+    cell_source_sink_solver_temp_export1.assign(ns, 0.0);
+    cell_source_sink_solver_temp_export2.assign(ns, 0.0);
+    for (size_t d = 0; d < ns; ++d) {
+        double t1 = secretion_rates[d];
+        t1 *= saturation_densities[d];
+        t1 *= f;
+        cell_source_sink_solver_temp1[d] = t1;
+        double t2 = std::fma(f, secretion_rates[d], 1.0);
+        cell_source_sink_solver_temp2[d] = std::fma(f, uptake_rates[d], t2);
+        const double e1 = net_export_rates[d] * dt;
+        cell_source_sink_solver_temp_export1[d] = e1;
+        cell_source_sink_solver_temp_export2[d] = e1 / W;
+    }
+    volume_is_changed = false;
+}
+
+void Basic_Agent::simulate_secretion_and_uptake(Microenvironment* m, double dt) {
+    if (!is_active) return;
+    if (volume_is_changed) set_internal_uptake_constants(dt);
+    double* rho = m->density_vector(current_voxel_index);
+    const int ns = m->number_of_densities();
+    for (int d = 0; d < ns; ++d) rho[d] += cell_source_sink_solver_temp1[d];
+    for (int d = 0; d < ns; ++d) rho[d] /= cell_source_sink_solver_temp2[d];
+    for (int d = 0; d < ns; ++d) rho[d] += cell_source_sink_solver_temp_export2[d];
+}
+
+// ---- microenvironment -------------------------------------------------------------------
+
+Microenvironment::Microenvironment() {
+    set_density(0, "unnamed", "none");
+    mesh.resize(-0.5, 0.5, -0.5, 0.5, -0.5, 0.5, 1, 1, 1);
+    host_field.assign((size_t)number_of_voxels() * number_of_densities(), 0.0);
+    diffusion_decay_solver = diffusion_decay_solver__constant_coefficients_LOD_3D_GPU;
+}
+
+Microenvironment::~Microenvironment() {
+    if (env) bfa_destroy(env);
+}
+
+void Microenvironment::set_density(int index, const std::string& nm, const std::string& units) {
+    if ((int)density_names.size() <= index) {
+        density_names.resize(index + 1, "unnamed");
+        density_units.resize(index + 1, "none");
+        diffusion_coefficients.resize(index + 1, 0.0);
+        decay_rates.resize(index + 1, 0.0);
+        dirichlet_activation_vector.resize(index + 1, false);
+        face_values.assign(6 * density_names.size(), 1.0);
+        host_field.assign((size_t)number_of_voxels() * density_names.size(), 0.0);
+        if (env) {
+            bfa_destroy(env);
+            env = nullptr;
+        }
+    }
+    density_names[index] = nm;
+    density_units[index] = units;
+    state_dirty = true;
+}
+
+void Microenvironment::add_density(const std::string& nm, const std::string& units, double D,
+                                   double decay) {
+    const int i = number_of_densities();
+    set_density(i, nm, units);
+    diffusion_coefficients[i] = D;
+    decay_rates[i] = decay;
+}
+
+void Microenvironment::resize_space_uniform(double x_start, double x_end, double y_start,
+                                            double y_end, double z_start, double z_end,
+                                            double dx_new) {
+    mesh.resize(x_start, x_end, y_start, y_end, z_start, z_end, dx_new, dx_new, dx_new);
+    host_field.assign((size_t)number_of_voxels() * number_of_densities(), 0.0);
+    if (env) {
+        bfa_destroy(env);
+        env = nullptr;
+    }
+    host_newer = true;
+    device_newer = false;
+    state_dirty = true;
+}
+
+int Microenvironment::ensure_device() {
+    if (env) return BFA_OK;
+    int rc = bfa_create(&env, mesh.x_nodes, mesh.y_nodes, mesh.z_nodes, number_of_densities(),
+                        mesh.dx, precision);
+    if (rc) {
+        last_error = bfa_last_error(nullptr);
+        env = nullptr;
+        return rc;
+    }
+    host_newer = true;
+    state_dirty = true;
+    agents_sig.clear();
+    return BFA_OK;
+}
+
+bfa_env* Microenvironment::device() { return ensure_device() ? nullptr : env; }
+
+int Microenvironment::push_state() {
+    int rc = ensure_device();
+    if (rc) return rc;
+    if (state_dirty) {
+        const int ns = number_of_densities();
+        std::vector<uint8_t> act(ns);
+        for (int d = 0; d < ns; ++d) act[d] = dirichlet_activation_vector[d] ? 1 : 0;
+        if ((rc = bfa_set_coefficients(env, diffusion_coefficients.data(), decay_rates.data())) ||
+            (rc = bfa_set_dirichlet(env, act.data(), face_registered, face_values.data()))) {
+            last_error = bfa_last_error(env);
+            return rc;
+        }
+        state_dirty = false;
+    }
+    if (host_newer) {
+        if ((rc = bfa_upload(env, host_field.data(), BFA_LAYOUT_REF))) {
+            last_error = bfa_last_error(env);
+            return rc;
+        }
+        host_newer = false;
+        device_newer = false;
+    }
+    return BFA_OK;
+}
+
+void Microenvironment::download_if_needed() {
+    if (!device_newer || !env) return;
+    if (bfa_download(env, host_field.data(), BFA_LAYOUT_REF)) {
+        std::cout << "Error: density download failed: " << bfa_last_error(env) << std::endl;
+        return;
+    }
+    device_newer = false;
+}
+
+double* Microenvironment::density_vector(int n) {
+    download_if_needed();
+    host_newer = true;
+    return host_field.data() + (size_t)n * number_of_densities();
+}
+
+std::vector<double>& Microenvironment::density_field() {
+    download_if_needed();
+    host_newer = true;
+    return host_field;
+}
+
+void Microenvironment::add_dirichlet_face(int face, const std::vector<double>& value) {
+    if (face < 0 || face > 5 || (int)value.size() != number_of_densities()) {
+        std::cout << "Error: add_dirichlet_face needs a face in 0..5 and one value per density"
+                  << std::endl;
+        return;
+    }
+    face_registered[face] = 1;
+    std::copy(value.begin(), value.end(), face_values.begin() + (size_t)face * number_of_densities());
+    state_dirty = true;
+}
+
+void Microenvironment::set_substrate_dirichlet_activation(int substrate_index, bool new_value) {
+    dirichlet_activation_vector[substrate_index] = new_value;
+    state_dirty = true;
+}
+
+void Microenvironment::apply_dirichlet_boundaries_conditions() {
+    if (push_state() || bfa_apply_dirichlet(env)) {
+        std::cout << "Error: apply_dirichlet_boundaries_conditions: "
+                  << (env ? bfa_last_error(env) : last_error.c_str()) << std::endl;
+        return;
+    }
+    device_newer = true;
+}
+
+void Microenvironment::simulate_diffusion_decay(double dt) {
+    // BioFVM/BioFVM_microenvironment.cpp:1132-1143
+    if (diffusion_decay_solver) {
+        diffusion_decay_solver(*this, dt);
+    } else {
+        std::cout << "Warning: diffusion-reaction-source/sink solver not set for Microenvironment object at "
+                  << this << ". Nothing happened!" << std::endl;
+    }
+}
+
+void Microenvironment::simulate_cell_sources_and_sinks(std::vector<Basic_Agent*>& agents, double dt) {
+    if (push_state()) {
+        std::cout << "Error: simulate_cell_sources_and_sinks: " << last_error << std::endl;
+        return;
+    }
+    const int ns = number_of_densities();
+    const int n = (int)agents.size();
+    bool changed = agents_sig.size() != agents.size();
+    for (int a = 0; a < n && !changed; ++a) changed = agents_sig[a] != agents[a];
+    std::vector<int32_t> vox(n);
+    std::vector<uint8_t> act(n);
+    std::vector<double> c1((size_t)n * ns), c2((size_t)n * ns), e2((size_t)n * ns);
+    for (int a = 0; a < n; ++a) {
+        Basic_Agent* ag = agents[a];
+        if (ag->volume_is_changed) {
+            ag->set_internal_uptake_constants(dt);
+            changed = true;
+        }
+        vox[a] = ag->current_voxel_index;
+        act[a] = ag->is_active ? 1 : 0;
+        for (int d = 0; d < ns; ++d) {
+            c1[(size_t)a * ns + d] = ag->cell_source_sink_solver_temp1[d];
+            c2[(size_t)a * ns + d] = ag->cell_source_sink_solver_temp2[d];
+            e2[(size_t)a * ns + d] = ag->cell_source_sink_solver_temp_export2[d];
+        }
+    }
+    // agents move and change: compare with what the device holds
+    changed = changed || vox != sig_voxel || act != sig_active || c1 != sig_c1 || c2 != sig_c2 ||
+              e2 != sig_e2;
+    if (changed) {
+        if (bfa_set_agents(env, n, vox.data(), act.data(), c1.data(), c2.data(), e2.data())) {
+            std::cout << "Error: set_agents: " << bfa_last_error(env) << std::endl;
+            return;
+        }
+        agents_sig.assign(agents.begin(), agents.end());
+        sig_voxel.swap(vox);
+        sig_active.swap(act);
+        sig_c1.swap(c1);
+        sig_c2.swap(c2);
+        sig_e2.swap(e2);
+    }
+    if (bfa_secretion_uptake(env)) {
+        std::cout << "Error: secretion/uptake: " << bfa_last_error(env) << std::endl;
+        return;
+    }
+    device_newer = true;
+}
+
+double Microenvironment::total_mass(int substrate) {
+    if (push_state()) return NAN;
+    std::vector<double> m(number_of_densities());
+    if (bfa_total_mass(env, m.data())) return NAN;
+    return m[substrate];
+}
+
+void diffusion_decay_solver__constant_coefficients_LOD_3D_GPU(Microenvironment& M, double dt) {
+    // the reference solver needs a uniform Cartesian mesh (BioFVM/BioFVM_solvers.cpp:116-121)
+    if (!M.mesh.uniform_mesh) {
+        std::cout << "Error: This algorithm is written for uniform Cartesian meshes. Try: other solvers!"
+                  << std::endl
+                  << std::endl;
+        return;
+    }
+    if (M.push_state()) {
+        std::cout << "Error: diffusion solver setup failed: " << M.last_error << std::endl;
+        return;
+    }
+    if (bfa_diffusion_decay(M.env, dt)) {
+        std::cout << "Error: diffusion solver failed: " << bfa_last_error(M.env) << std::endl;
+        return;
+    }
+    M.device_newer = true;
+}
+
+}  // namespace BioFVM_amd

@@ -1,0 +1,144 @@
+// BioFVM_amd.h -- C++ host side of the MI355X diffusion-decay path.
+//
+// Mirrors the part of the reference's BioFVM interface that the LOD
+// diffusion-decay path uses, with the same names, argument meaning and error
+// behaviour, and runs the compute through the C ABI of include/biofvm_amd.h:
+//
+//   Cartesian_Mesh        BioFVM/BioFVM_mesh.h:215-301 (uniform mesh, world size 1)
+//   Microenvironment      BioFVM/BioFVM_microenvironment.h:71-373
+//     diffusion_decay_solver function pointer          (.h:199)
+//     simulate_diffusion_decay(dt)                     (.cpp:1132-1143)
+//     simulate_cell_sources_and_sinks(agents, dt)      (.cpp:1370-1384)
+//     apply_dirichlet_boundaries_conditions()          (.cpp:348-446)
+//     add_dirichlet_node / set_substrate_dirichlet_activation (.cpp:218-273)
+//   Basic_Agent           BioFVM/BioFVM_basic_agent.h
+//     set_internal_uptake_constants(dt)                (.cpp:224-274)
+//     simulate_secretion_and_uptake(M, dt)             (.cpp:509-559)
+//
+// The density field lives on the GPU between steps.  density_vector() and
+// the other host accessors download it when the device copy is newer, and
+// mark it to be uploaded before the next device step when the caller writes
+// through them.  Solver errors are printed and the call returns, as the
+// reference does (BioFVM/BioFVM_solvers.cpp:116-121).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "biofvm_amd.h"
+
+namespace BioFVM_amd {
+
+class Microenvironment;
+
+struct Cartesian_Mesh {
+    // {x_min, y_min, z_min, x_max, y_max, z_max}, as bounding_box in the reference
+    double bounding_box[6] = {0, 0, 0, 1, 1, 1};
+    double dx = 1, dy = 1, dz = 1, dV = 1;
+    int x_nodes = 1, y_nodes = 1, z_nodes = 1;
+    bool uniform_mesh = true;
+
+    // Cartesian_Mesh::resize (BioFVM/BioFVM_mesh.cpp:1436-1662) at world size 1
+    void resize(double x_start, double x_end, double y_start, double y_end, double z_start,
+                double z_end, double dx_new, double dy_new, double dz_new);
+    int voxel_index(int i, int j, int k) const { return (i * y_nodes + j) * z_nodes + k; }
+    int number_of_voxels() const { return x_nodes * y_nodes * z_nodes; }
+    bool is_position_valid(double x, double y, double z) const;
+    int nearest_voxel_index(const std::vector<double>& position) const;
+};
+
+class Basic_Agent {
+  public:
+    std::vector<double> position = {0, 0, 0};
+    double volume = 2494.0;
+    bool is_active = true;
+    int current_voxel_index = -1;
+    std::vector<double> secretion_rates, uptake_rates, saturation_densities, net_export_rates;
+    // constants of the implicit source/sink update (BioFVM/BioFVM_basic_agent.cpp:241-265)
+    std::vector<double> cell_source_sink_solver_temp1, cell_source_sink_solver_temp2;
+    std::vector<double> cell_source_sink_solver_temp_export1, cell_source_sink_solver_temp_export2;
+    bool volume_is_changed = true;
+
+    void register_microenvironment(Microenvironment* m);
+    void update_voxel_index();
+    void set_internal_uptake_constants(double dt);
+    // One agent's update on the host copy of the field (the reference's
+    // per-agent call).  Microenvironment::simulate_cell_sources_and_sinks
+    // does all agents at once on the GPU instead.
+    void simulate_secretion_and_uptake(Microenvironment* m, double dt);
+
+  private:
+    Microenvironment* microenvironment = nullptr;
+};
+
+class Microenvironment {
+  public:
+    Cartesian_Mesh mesh;
+    std::vector<std::string> density_names, density_units;
+    std::vector<double> diffusion_coefficients, decay_rates;
+    std::vector<bool> dirichlet_activation_vector;
+    std::string name = "unnamed";
+    int precision = BFA_FP64;
+
+    void (*diffusion_decay_solver)(Microenvironment&, double) = nullptr;
+
+    Microenvironment();
+    ~Microenvironment();
+    Microenvironment(const Microenvironment&) = delete;
+    Microenvironment& operator=(const Microenvironment&) = delete;
+
+    // densities and space
+    void set_density(int index, const std::string& name, const std::string& units);
+    void add_density(const std::string& name, const std::string& units, double D, double decay);
+    int number_of_densities() const { return (int)density_names.size(); }
+    int number_of_voxels() const { return mesh.number_of_voxels(); }
+    void resize_space_uniform(double x_start, double x_end, double y_start, double y_end,
+                              double z_start, double z_end, double dx);
+
+    // field access in the reference layout ((i*ny + j)*nz + k)*ns + d
+    double* density_vector(int n);
+    double* density_vector(int i, int j, int k) { return density_vector(mesh.voxel_index(i, j, k)); }
+    std::vector<double>& density_field();  // whole field (host copy, synchronised)
+
+    // Dirichlet state (BioFVM/BioFVM_microenvironment.cpp:218-273,632)
+    void add_dirichlet_face(int face, const std::vector<double>& value);  // face 0..5 = xmin..zmax
+    void set_substrate_dirichlet_activation(int substrate_index, bool new_value);
+    void apply_dirichlet_boundaries_conditions();
+
+    // stepping
+    void simulate_diffusion_decay(double dt);
+    void simulate_cell_sources_and_sinks(std::vector<Basic_Agent*>& agents, double dt);
+    double total_mass(int substrate);
+
+    // device plumbing
+    bfa_env* device();
+    void download_if_needed();
+    void mark_host_modified() { host_newer = true; }
+    void invalidate_agents() { agents_sig.clear(); }
+    std::string last_error;
+
+  private:
+    friend void diffusion_decay_solver__constant_coefficients_LOD_3D_GPU(Microenvironment&, double);
+    int ensure_device();
+    int push_state();
+    std::vector<double> host_field;
+    bfa_env* env = nullptr;
+    bool host_newer = true;    // host copy was written since the last upload
+    bool device_newer = false; // device copy was stepped since the last download
+    bool state_dirty = true;   // coefficients / Dirichlet changed
+    unsigned char face_registered[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<double> face_values;  // [6][ns]
+    // agent state last sent to the device
+    std::vector<const Basic_Agent*> agents_sig;
+    std::vector<int32_t> sig_voxel;
+    std::vector<uint8_t> sig_active;
+    std::vector<double> sig_c1, sig_c2, sig_e2;
+};
+
+// The drop-in solver: assign it to Microenvironment::diffusion_decay_solver
+// (the reference assigns diffusion_decay_solver__constant_coefficients_LOD_3D
+// or _LOD_3D_BLOCKING at BioFVM/BioFVM_microenvironment.cpp:1943,2506).
+void diffusion_decay_solver__constant_coefficients_LOD_3D_GPU(Microenvironment& M, double dt);
+
+}  // namespace BioFVM_amd

@@ -1,0 +1,93 @@
+// Reference-style driver of the C++ host side (test infrastructure).
+//
+// Reads a case file written by tests/test_gpu_adapter.py, sets the
+// microenvironment up the way initialize_microenvironment does (densities,
+// uniform mesh, initial field, Dirichlet faces, activation), registers
+// agents, and runs the main-loop pair of the reference (main.cpp:244 then
+// main.cpp:259): simulate_diffusion_decay(dt) then the agents' secretion and
+// uptake.  Writes the final field (reference layout) and the total mass of
+// every substrate after every step.
+//
+// Case file (little endian): int32 {nx, ny, nz, ns, steps, n_agents,
+// precision}; float64 {dx, dt, D[ns], decay[ns], active[ns], registered[6],
+// face_values[6*ns], field[nx*ny*nz*ns], then per agent: position[3],
+// volume, secretion[ns], uptake[ns], export[ns], saturation[ns]}.
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <vector>
+
+#include "BioFVM_amd.h"
+
+using namespace BioFVM_amd;
+
+int main(int argc, char** argv) {
+    if (argc != 3) {
+        std::cerr << "usage: adapter_demo case.bin out.bin" << std::endl;
+        return 2;
+    }
+    std::ifstream in(argv[1], std::ios::binary);
+    int32_t hdr[7];
+    in.read(reinterpret_cast<char*>(hdr), sizeof hdr);
+    const int nx = hdr[0], ny = hdr[1], nz = hdr[2], ns = hdr[3], steps = hdr[4], na = hdr[5];
+    auto rd = [&](size_t n) {
+        std::vector<double> v(n);
+        in.read(reinterpret_cast<char*>(v.data()), n * sizeof(double));
+        return v;
+    };
+    const double dx = rd(1)[0], dt = rd(1)[0];
+    std::vector<double> D = rd(ns), decay = rd(ns), active = rd(ns), reg = rd(6), fv = rd(6 * (size_t)ns);
+    std::vector<double> field = rd((size_t)nx * ny * nz * ns);
+    if (!in) {
+        std::cerr << "short case file" << std::endl;
+        return 2;
+    }
+
+    Microenvironment M;
+    M.precision = hdr[6];
+    // as initialize_microenvironment: density 0 is set, the others added
+    M.set_density(0, "oxygen", "mmHg");
+    M.diffusion_coefficients[0] = D[0];
+    M.decay_rates[0] = decay[0];
+    for (int d = 1; d < ns; ++d) M.add_density("s" + std::to_string(d), "dimensionless", D[d], decay[d]);
+    M.resize_space_uniform(-0.5 * dx * nx, 0.5 * dx * nx, -0.5 * dx * ny, 0.5 * dx * ny,
+                           -0.5 * dx * nz, 0.5 * dx * nz, dx);
+    std::vector<double>& u = M.density_field();
+    if (u.size() != field.size()) {
+        std::cerr << "size mismatch " << u.size() << " vs " << field.size() << std::endl;
+        return 2;
+    }
+    u = field;
+    for (int f = 0; f < 6; ++f)
+        if (reg[f] != 0.0) M.add_dirichlet_face(f, std::vector<double>(fv.begin() + f * ns, fv.begin() + (f + 1) * ns));
+    for (int d = 0; d < ns; ++d) M.set_substrate_dirichlet_activation(d, active[d] != 0.0);
+
+    std::vector<Basic_Agent> agents(na);
+    std::vector<Basic_Agent*> list;
+    for (int a = 0; a < na; ++a) {
+        Basic_Agent& ag = agents[a];
+        ag.register_microenvironment(&M);
+        std::vector<double> p = rd(3);
+        ag.position = p;
+        ag.volume = rd(1)[0];
+        ag.secretion_rates = rd(ns);
+        ag.uptake_rates = rd(ns);
+        ag.net_export_rates = rd(ns);
+        ag.saturation_densities = rd(ns);
+        ag.update_voxel_index();
+        list.push_back(&ag);
+    }
+    std::vector<double> masses;
+    for (int s = 0; s < steps; ++s) {
+        M.simulate_diffusion_decay(dt);
+        M.simulate_cell_sources_and_sinks(list, dt);
+        for (int d = 0; d < ns; ++d) masses.push_back(M.total_mass(d));
+    }
+    const std::vector<double>& out = M.density_field();
+    std::ofstream o(argv[2], std::ios::binary);
+    o.write(reinterpret_cast<const char*>(out.data()), out.size() * sizeof(double));
+    o.write(reinterpret_cast<const char*>(masses.data()), masses.size() * sizeof(double));
+    std::cout << "adapter_demo: " << steps << " steps of " << nx << "x" << ny << "x" << nz << "x" << ns
+              << " with " << na << " agents done" << std::endl;
+    return 0;
+}

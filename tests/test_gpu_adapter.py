@@ -1,0 +1,70 @@
+"""The C++ host side (physicell-x_amd/host), driven like the reference's main
+loop by tests/cpp/adapter_demo.cpp, against the CPU oracle: FP64 <= 1e-12
+relative max-norm, total mass per step <= 1e-12 relative."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import ROOT, relmax
+
+pytestmark = pytest.mark.gpu
+
+DEMO = os.path.join(ROOT, "tests", "cpp", "adapter_demo")
+
+
+def write_case(path, nx, ny, nz, ns, steps, dx, dt, D, decay, active, reg, fv, field, agents, precision):
+    with open(path, "wb") as f:
+        np.array([nx, ny, nz, ns, steps, len(agents["V"]), precision], np.int32).tofile(f)
+        parts = [np.array([dx, dt]), D, decay, active.astype(float), reg.astype(float), fv.ravel(), field]
+        for p in parts:
+            np.asarray(p, np.float64).tofile(f)
+        for a in range(len(agents["V"])):
+            np.concatenate([agents["pos"][a], [agents["V"][a]], agents["S"][a], agents["U"][a],
+                            agents["E"][a], agents["rs"][a]]).astype(np.float64).tofile(f)
+
+
+@pytest.mark.parametrize("precision,tol", [(0, 1e-12), (1, 1e-5)])
+def test_cpp_host_side_matches_oracle(tmp_path, precision, tol):
+    assert os.path.exists(DEMO), "build the C++ host side first (make -C physicell-x_amd/host)"
+    rng = np.random.default_rng(31)
+    nx, ny, nz, ns, steps, dx, dt = 30, 26, 34, 2, 6, 20.0, 0.01
+    D = 10 ** rng.uniform(3, 5, ns)
+    decay = rng.uniform(0.01, 1.0, ns)
+    active = np.array([1, 0], np.uint8)
+    reg = np.array([1, 1, 1, 0, 1, 1], np.uint8)
+    fv = rng.uniform(0, 40, (6, ns))
+    field = (fv[0] + rng.standard_normal((nx, ny, nz, ns))).ravel()
+    na = 300
+    half = 0.5 * dx * np.array([nx, ny, nz])
+    pos = rng.uniform(-half, half, (na, 3))
+    pos[5] = pos[6] = pos[7]
+    pos[9] = half * 1.5  # outside the domain: deactivated
+    agents = dict(pos=pos, V=rng.uniform(1000, 4000, na), S=rng.uniform(0, 10, (na, ns)),
+                  U=rng.uniform(0, 10, (na, ns)), E=rng.uniform(0, 20, (na, ns)),
+                  rs=rng.uniform(0, 40, (na, ns)))
+    case = tmp_path / "case.bin"
+    out = tmp_path / "out.bin"
+    write_case(case, nx, ny, nz, ns, steps, dx, dt, D, decay, active, reg, fv, field, agents, precision)
+    r = subprocess.run([DEMO, str(case), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    res = np.fromfile(out, np.float64)
+    got, masses = res[: field.size], res[field.size:].reshape(steps, ns)
+    # oracle: same mesh, same voxel mapping, same constants
+    bbox = [-half[0], -half[1], -half[2], half[0], half[1], half[2]]
+    vox = oracle.voxel_indices(pos, bbox, dx, dx, dx, nx, ny, nz)
+    c1, c2, e1, e2 = oracle.agent_constants(dt, dx ** 3, agents["V"], agents["S"], agents["U"], agents["E"],
+                                            agents["rs"])
+    act = (vox >= 0).astype(np.uint8)
+    ag = oracle.AgentSet(np.where(vox >= 0, vox, 0), c1, c2, e1, e2, act)
+    orc = oracle.Oracle(nx, ny, nz, ns, dx, D, decay, active, reg, fv)
+    u = field.copy()
+    for s in range(steps):
+        orc.diffusion_step(u, dt)
+        orc.secretion_uptake(u, ag)
+        m = orc.total_mass(u)
+        assert np.max(np.abs(masses[s] - m) / np.abs(m)) <= max(tol, 1e-12), s
+    g, uu = got.reshape(-1, ns), u.reshape(-1, ns)
+    assert max(relmax(g[:, d], uu[:, d]) for d in range(ns)) <= tol
