@@ -68,11 +68,10 @@ struct bfa_env {
     int nag = 0, nseg = 0;
     int* seg_off = nullptr;
     long long* seg_vox = nullptr;
-    void* agA = nullptr;  // FP64: c1   FP32: A
-    void* agB = nullptr;  // FP64: c2   FP32: B
-    void* agC = nullptr;  // FP64: e2
-    std::vector<double> h_c1, h_c2, h_e2;  // sorted, [ns][nag], for FP32 rebuilds
-    bool ag_f32_valid = false;
+    void* agA = nullptr;  // c1 (FP64), sorted [ns][nag]
+    void* agB = nullptr;  // c2 (FP64)
+    void* agC = nullptr;  // e2 (FP64)
+    std::vector<double> h_c1, h_c2, h_e2;  // host copies of the sorted constants
 
     // graph of one step
     hipGraph_t graph = nullptr;
@@ -464,8 +463,9 @@ int launch_agents(bfa_env* e) {
     if (e->prec == BFA_FP32) {
         hipLaunchKernelGGL(bfa::k_agents_f32, dim3(blocks), dim3(256), 0, e->stream,
                            reinterpret_cast<float*>(e->u), e->N, e->ns, e->nseg, e->nag,
-                           e->seg_off, e->seg_vox, reinterpret_cast<const float*>(e->agA),
-                           reinterpret_cast<const float*>(e->agB));
+                           e->seg_off, e->seg_vox, reinterpret_cast<const double*>(e->agA),
+                           reinterpret_cast<const double*>(e->agB),
+                           reinterpret_cast<const double*>(e->agC), e->params);
     } else {
         hipLaunchKernelGGL(bfa::k_agents_f64, dim3(blocks), dim3(256), 0, e->stream,
                            reinterpret_cast<double*>(e->u), e->N, e->ns, e->nseg, e->nag,
@@ -525,8 +525,6 @@ int run_precompute(bfa_env* e, double dt) {
     return BFA_OK;
 }
 
-int rebuild_f32_agents(bfa_env* e);
-
 int ensure_precompute(bfa_env* e, double dt) {
     if (!e->coef_set) return fail(e, BFA_ERR_STATE, "bfa_set_coefficients has not been called");
     if (!(dt > 0.0) || !std::isfinite(dt)) return fail(e, BFA_ERR_ARG, "dt must be positive, got %g", dt);
@@ -577,27 +575,6 @@ int build_graph(bfa_env* e, double dt) {
     }
     e->graph_dt = dt;
     e->graph_valid = true;
-    return BFA_OK;
-}
-
-// FP32 agent map v <- A v + B from the FP64 constants and the basis.
-int rebuild_f32_agents(bfa_env* e) {
-    if (e->prec != BFA_FP32 || e->nag == 0) {
-        e->ag_f32_valid = true;
-        return BFA_OK;
-    }
-    const size_t cnt = (size_t)e->ns * e->nag;
-    std::vector<float> A(cnt), B(cnt);
-    for (int d = 0; d < e->ns; ++d)
-        for (int a = 0; a < e->nag; ++a) {
-            const size_t o = (size_t)d * e->nag + a;
-            const double c1 = e->h_c1[o], c2 = e->h_c2[o], e2 = e->h_e2[o];
-            A[o] = (float)(1.0 / c2);
-            B[o] = (float)((((e->b[d] + c1) / c2) + e2 - e->b[d]) / e->s[d]);
-        }
-    HIP_TRY(e, hipMemcpy(e->agA, A.data(), cnt * sizeof(float), hipMemcpyHostToDevice));
-    HIP_TRY(e, hipMemcpy(e->agB, B.data(), cnt * sizeof(float), hipMemcpyHostToDevice));
-    e->ag_f32_valid = true;
     return BFA_OK;
 }
 
@@ -715,6 +692,16 @@ int bfa_create(bfa_env** out, int nx, int ny, int nz, int ns, double dx, int pre
         (rc = dalloc(e, &e->dvalT, 7 * (size_t)ns * e->tsize)) ||
         (rc = dalloc(e, &e->params, 5 * (size_t)ns * 8)))
         return bail(rc);
+    {
+        // neutral parameters until the first upload / precompute: b = 0, s = 1
+        std::vector<double> p0(5 * (size_t)ns, 0.0);
+        for (int d = 0; d < ns; ++d) p0[3 * ns + d] = p0[4 * ns + d] = 1.0;
+        st = hipMemcpy(e->params, p0.data(), p0.size() * 8, hipMemcpyHostToDevice);
+        if (st != hipSuccess) {
+            fail(e, BFA_ERR_HIP, "params init failed: %s", hipGetErrorString(st));
+            return bail(BFA_ERR_HIP);
+        }
+    }
     e->red_blocks = (int)std::min<long long>(512, std::max<long long>(1, e->N / 8192));
     if ((rc = dalloc(e, &e->red_partial, (size_t)e->red_blocks * ns * 8)) ||
         (rc = dalloc(e, &e->red_out, (size_t)ns * 8)))
@@ -866,10 +853,7 @@ int bfa_upload(bfa_env* e, const double* field, int layout) {
             }
         }
     }
-    if (e->prec == BFA_FP32) {
-        e->pre_valid = false;  // beta and the Dirichlet table depend on the basis
-        e->ag_f32_valid = false;
-    }
+    if (e->prec == BFA_FP32) e->pre_valid = false;  // beta and the Dirichlet table follow the basis
     // params buffer: b at [2ns], 1/s at [4ns] (and s at [3ns]) for the conversion
     std::vector<double> p(5 * (size_t)e->ns, 0.0);
     for (int d = 0; d < e->ns; ++d) {
@@ -1023,28 +1007,18 @@ int bfa_set_agents(bfa_env* e, int n, const int32_t* voxel, const uint8_t* activ
     HIP_TRY(e, hipMemcpy(e->seg_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(e, hipMemcpy(e->seg_vox, svox.data(), svox.size() * sizeof(long long), hipMemcpyHostToDevice));
     const size_t cnt = (size_t)ns * m;
-    if (e->prec == BFA_FP64) {
-        if ((rc = dalloc(e, &e->agA, cnt * 8)) || (rc = dalloc(e, &e->agB, cnt * 8)) ||
-            (rc = dalloc(e, &e->agC, cnt * 8)))
-            return rc;
-        HIP_TRY(e, hipMemcpy(e->agA, e->h_c1.data(), cnt * 8, hipMemcpyHostToDevice));
-        HIP_TRY(e, hipMemcpy(e->agB, e->h_c2.data(), cnt * 8, hipMemcpyHostToDevice));
-        HIP_TRY(e, hipMemcpy(e->agC, e->h_e2.data(), cnt * 8, hipMemcpyHostToDevice));
-    } else {
-        if ((rc = dalloc(e, &e->agA, cnt * 4)) || (rc = dalloc(e, &e->agB, cnt * 4))) return rc;
-    }
+    if ((rc = dalloc(e, &e->agA, cnt * 8)) || (rc = dalloc(e, &e->agB, cnt * 8)) ||
+        (rc = dalloc(e, &e->agC, cnt * 8)))
+        return rc;
+    HIP_TRY(e, hipMemcpy(e->agA, e->h_c1.data(), cnt * 8, hipMemcpyHostToDevice));
+    HIP_TRY(e, hipMemcpy(e->agB, e->h_c2.data(), cnt * 8, hipMemcpyHostToDevice));
+    HIP_TRY(e, hipMemcpy(e->agC, e->h_e2.data(), cnt * 8, hipMemcpyHostToDevice));
     e->nag = m;
     e->nseg = nseg;
-    e->ag_f32_valid = false;
-    return rebuild_f32_agents(e);
-}
-
-static int ready(bfa_env* e, double dt) {
-    int rc = ensure_precompute(e, dt);
-    if (rc) return rc;
-    if (!e->ag_f32_valid && (rc = rebuild_f32_agents(e))) return rc;
     return BFA_OK;
 }
+
+static int ready(bfa_env* e, double dt) { return ensure_precompute(e, dt); }
 
 int bfa_diffusion_decay(bfa_env* e, double dt) {
     if (!e) return BFA_ERR_ARG;
@@ -1055,8 +1029,6 @@ int bfa_diffusion_decay(bfa_env* e, double dt) {
 
 int bfa_secretion_uptake(bfa_env* e) {
     if (!e) return BFA_ERR_ARG;
-    int rc;
-    if (!e->ag_f32_valid && (rc = rebuild_f32_agents(e))) return rc;
     return launch_agents(e);
 }
 

@@ -65,8 +65,10 @@ __global__ void k_thomas(int n, int ns, const double* __restrict__ params, doubl
 // the reference's serial loop order (BioFVM/BioFVM_microenvironment.cpp:
 // 1370-1384) with the per-agent update of BioFVM/BioFVM_basic_agent.cpp:
 // 536-552.  FP64: rho = (rho + c1)/c2 + e2 in the reference's operation
-// order.  FP32 deviation form: v = A*v + B with A = 1/c2 and
-// B = ((b + c1)/c2 + e2 - b)/s precomputed in FP64.
+// order.  FP32 deviation form: the stored v is lifted to u = b + s*v in FP64,
+// updated with the same FP64 operations and stored back as (u - b)/s, so the
+// agent maps carry no FP32-rounded coefficients (whose fixed error would be
+// amplified near the map's fixed point).
 // Constants are stored [ns][nagents_sorted] so lanes of one substrate read
 // consecutive entries.
 __global__ void k_agents_f64(double* __restrict__ u, long long N, int ns, int nseg,
@@ -91,17 +93,23 @@ __global__ void k_agents_f64(double* __restrict__ u, long long N, int ns, int ns
 
 __global__ void k_agents_f32(float* __restrict__ u, long long N, int ns, int nseg, int nag,
                              const int* __restrict__ off, const long long* __restrict__ vox,
-                             const float* __restrict__ A, const float* __restrict__ B) {
+                             const double* __restrict__ c1, const double* __restrict__ c2,
+                             const double* __restrict__ e2, const double* __restrict__ params) {
     const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= (long long)nseg * ns) return;
     const int d = (int)(gid / nseg);
     const int q = (int)(gid - (long long)d * nseg);
+    const double b = params[2 * ns + d], s = params[3 * ns + d], rs = params[4 * ns + d];
     float* p = u + (long long)d * N + vox[q];
-    float v = *p;
+    double rho = fma((double)*p, s, b);
     const int a0 = off[q], a1 = off[q + 1];
     const size_t o = (size_t)d * nag;
-    for (int a = a0; a < a1; ++a) v = fmaf(A[o + a], v, B[o + a]);
-    *p = v;
+    for (int a = a0; a < a1; ++a) {
+        rho += c1[o + a];
+        rho /= c2[o + a];
+        rho += e2[o + a];
+    }
+    *p = (float)((rho - b) * rs);
 }
 
 // ---- standalone Dirichlet pass over the six faces ---------------------------

@@ -31,6 +31,7 @@ struct EnvDesc {
     int nx, ny, nz, ns;
     unsigned reg;
     unsigned long long active;
+    const double* params;  // [5][ns]: D, decay, b, s, 1/s (FP32 deviation basis)
     int steps;  // steps of the current run this member takes; items of later steps are skipped
     int pad;
 };
@@ -64,88 +65,6 @@ __device__ __forceinline__ SweepArgs<T> args_of(const EnvDesc<T>& e, int axis) {
     a.first = 0;
     a.count = 0;
     return a;
-}
-
-// One row group of the wave-local x sweep (see k_sweep_x_wave): rows r..r+RW
-// share the wavefront, lane p of each row owns nodes [p*L, p*L+L).
-template <typename T, int L, bool DEV, bool VEC>
-__device__ __forceinline__ void x_row_group(const SweepArgs<T>& a, int d, int r, bool have,
-                                            int p, int P, int s, int valid, const T (&inv)[L],
-                                            const T (&cp)[L]) {
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
-    T* ub = a.u + (long long)d * a.N;
-    const T k = a.k[d];
-    const bool dact = (a.active >> d) & 1ull;
-    T cur[L];
-    if (have)
-        load_chunk<T, L, VEC>(cur, ub + (long long)r * nx + s, valid);
-    else {
-#pragma unroll
-        for (int m = 0; m < L; ++m) cur[m] = T(0);
-    }
-    T v_lo = T(0), v_hi = T(0), v_in = T(0);
-    bool rowb = false;
-    const bool dir_on = have && dact;
-    if (dir_on) {
-        const int kk = r / ny;
-        const int jj = r - kk * ny;
-        rowb = (jj == 0 || jj == ny - 1 || kk == 0 || kk == nz - 1);
-        v_lo = a.dval[dir_slot(0, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
-        v_hi = a.dval[dir_slot(nx - 1, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
-        v_in = a.dval[dir_slot(-1, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
-        dirichlet_row<T, L>(cur, s, nx, rowb, v_lo, v_hi, v_in);
-    }
-    if (DEV) {
-        const T bt = a.beta[d];
-#pragma unroll
-        for (int m = 0; m < L; ++m) cur[m] = (m < valid) ? cur[m] - bt : T(0);
-    }
-    T za = T(1), zb = T(0);
-#pragma unroll
-    for (int m = 0; m < L; ++m) {
-        zb = fma(k, zb, cur[m]) * inv[m];
-        za = za * (k * inv[m]);
-    }
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const T pa = shfl_up_t(za, off), pb = shfl_up_t(zb, off);
-        if (off < P && p >= off) {
-            zb = fma(za, pb, zb);
-            za = za * pa;
-        }
-    }
-    T y = shfl_up_t(zb, 1);
-    if (p == 0) y = T(0);
-#pragma unroll
-    for (int m = 0; m < L; ++m) {
-        y = fma(k, y, cur[m]) * inv[m];
-        cur[m] = y;
-    }
-    T wa = T(1), wb = T(0);
-#pragma unroll
-    for (int m = L - 1; m >= 0; --m) {
-        wb = fma(-cp[m], wb, cur[m]);
-        wa = wa * (-cp[m]);
-    }
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const T na = shfl_down_t(wa, off), nb = shfl_down_t(wb, off);
-        if (off < P && p + off < P) {
-            wb = fma(wa, nb, wb);
-            wa = wa * na;
-        }
-    }
-    T xn = shfl_down_t(wb, 1);
-    if (p == P - 1) xn = T(0);
-#pragma unroll
-    for (int m = L - 1; m >= 0; --m) {
-        xn = fma(-cp[m], xn, cur[m]);
-        cur[m] = xn;
-    }
-    if (have) {
-        if (dir_on) dirichlet_row<T, L>(cur, s, nx, rowb, v_lo, v_hi, v_in);
-        store_chunk<T, L, VEC>(cur, ub + (long long)r * nx + s, valid);
-    }
 }
 
 // Persistent batched x sweep.  Each wavefront takes XItems from ticket[0]
@@ -277,11 +196,12 @@ __global__ __launch_bounds__(1024) void kb_sweep_col(const EnvDesc<T>* __restric
 
 // Agent entries over the whole batch: entry q updates the value at absolute
 // device address u_of[q] with its agents' maps a0..a1 (ascending agent
-// order).  FP64: rho = (rho + c1)/c2 + e2; FP32: v = A*v + B.
+// order): rho = (rho + c1)/c2 + e2 in FP64; FP32 members lift v to
+// u = b + s*v in FP64 for the update and store (u - b)/s.
 struct AgentEntry {
     unsigned long long addr;  // element address
     int a0, a1;
-    int env, pad;
+    int env, d;
 };
 
 template <typename T>
@@ -306,15 +226,23 @@ __global__ void kb_agents_f64(const EnvDesc<T>* __restrict__ envs,
 template <typename T>
 __global__ void kb_agents_f32(const EnvDesc<T>* __restrict__ envs,
                               const AgentEntry* __restrict__ ent, const int* __restrict__ nent,
-                              const float* __restrict__ A, const float* __restrict__ B, int step) {
+                              const double* __restrict__ c1, const double* __restrict__ c2,
+                              const double* __restrict__ e2, int step) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= *nent) return;
     const AgentEntry en = ent[q];
-    if (step >= envs[en.env].steps) return;
+    const EnvDesc<T>& e = envs[en.env];
+    if (step >= e.steps) return;
+    const double b = e.params[2 * e.ns + en.d], s = e.params[3 * e.ns + en.d],
+                 rs = e.params[4 * e.ns + en.d];
     float* p = reinterpret_cast<float*>(en.addr);
-    float v = *p;
-    for (int a = en.a0; a < en.a1; ++a) v = fmaf(A[a], v, B[a]);
-    *p = v;
+    double rho = fma((double)*p, s, b);
+    for (int a = en.a0; a < en.a1; ++a) {
+        rho += c1[a];
+        rho /= c2[a];
+        rho += e2[a];
+    }
+    *p = (float)((rho - b) * rs);
 }
 
 // Zero `n` ints (graph-safe replacement for a memset node).

@@ -138,6 +138,7 @@ __device__ __forceinline__ void chunk_solve(T (&x)[L], bool act, int p, int P, i
     __syncthreads();
     if (act) {
         T y = T(0);
+#pragma unroll 8
         for (int q = 0; q < p; ++q) y = fma(s_g[q], y, s_z[q * lines + line]);
         fwd_apply<T, L>(x, s, s_inv, k, y);
     }
@@ -150,6 +151,7 @@ __device__ __forceinline__ void chunk_solve(T (&x)[L], bool act, int p, int P, i
     __syncthreads();
     if (act) {
         T xn = T(0);
+#pragma unroll 8
         for (int q = P - 1; q > p; --q) xn = fma(s_g[q], xn, s_z[q * lines + line]);
         bwd_apply<T, L>(x, s, s_cp, xn);
     }
@@ -337,9 +339,100 @@ __device__ __forceinline__ T shfl_up_t(T v, int d) { return __shfl_up(v, d, 64);
 template <typename T>
 __device__ __forceinline__ T shfl_down_t(T v, int d) { return __shfl_down(v, d, 64); }
 
+// One row group of the wave-local x sweep: rows r..r+RW share the
+// wavefront, lane p of each row owns nodes [p*L, p*L+L) of row r (its
+// 1/den and c' in inv/cp, zero past the row end).  Dirichlet passes D0 and
+// D1 are applied to the row's end nodes, and to all its nodes when the row
+// lies on a y or z face.  Used by the single-microenvironment and the batch
+// x sweeps.
+template <typename T, int L, bool DEV, bool VEC>
+__device__ __forceinline__ void x_row_group(const SweepArgs<T>& a, int d, int r, bool have,
+                                            int p, int P, int s, int valid, const T (&inv)[L],
+                                            const T (&cp)[L]) {
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    T* ub = a.u + (long long)d * a.N;
+    const T k = a.k[d];
+    const bool dact = (a.active >> d) & 1ull;
+    T cur[L];
+    if (have)
+        load_chunk<T, L, VEC>(cur, ub + (long long)r * nx + s, valid);
+    else {
+#pragma unroll
+        for (int m = 0; m < L; ++m) cur[m] = T(0);
+    }
+    // lanes holding a row end, or every lane of a face row, need Dirichlet
+    // values; the others skip the lookups
+    const bool has_end = s == 0 || s + valid == nx;
+    T v_lo = T(0), v_hi = T(0), v_in = T(0);
+    bool rowb = false, dir_on = false;
+    if (have && dact) {
+        const int kk = r / ny;
+        const int jj = r - kk * ny;
+        rowb = (jj == 0 || jj == ny - 1 || kk == 0 || kk == nz - 1);
+        dir_on = rowb || has_end;
+        if (dir_on) {
+            v_lo = a.dval[dir_slot(0, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
+            v_hi = a.dval[dir_slot(nx - 1, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
+            if (rowb) v_in = a.dval[dir_slot(-1, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
+            dirichlet_row<T, L>(cur, s, nx, rowb, v_lo, v_hi, v_in);
+        }
+    }
+    if (DEV) {
+        const T bt = a.beta[d];
+#pragma unroll
+        for (int m = 0; m < L; ++m) cur[m] = (m < valid) ? cur[m] - bt : T(0);
+    }
+    T za = T(1), zb = T(0);
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+        zb = fma(k, zb, cur[m]) * inv[m];
+        za = za * (k * inv[m]);
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const T pa = shfl_up_t(za, off), pb = shfl_up_t(zb, off);
+        if (off < P && p >= off) {
+            zb = fma(za, pb, zb);
+            za = za * pa;
+        }
+    }
+    T y = shfl_up_t(zb, 1);
+    if (p == 0) y = T(0);
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+        y = fma(k, y, cur[m]) * inv[m];
+        cur[m] = y;
+    }
+    T wa = T(1), wb = T(0);
+#pragma unroll
+    for (int m = L - 1; m >= 0; --m) {
+        wb = fma(-cp[m], wb, cur[m]);
+        wa = wa * (-cp[m]);
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const T na = shfl_down_t(wa, off), nb = shfl_down_t(wb, off);
+        if (off < P && p + off < P) {
+            wb = fma(wa, nb, wb);
+            wa = wa * na;
+        }
+    }
+    T xn = shfl_down_t(wb, 1);
+    if (p == P - 1) xn = T(0);
+#pragma unroll
+    for (int m = L - 1; m >= 0; --m) {
+        xn = fma(-cp[m], xn, cur[m]);
+        cur[m] = xn;
+    }
+    if (have) {
+        if (dir_on) dirichlet_row<T, L>(cur, s, nx, rowb, v_lo, v_hi, v_in);
+        store_chunk<T, L, VEC>(cur, ub + (long long)r * nx + s, valid);
+    }
+}
+
 template <typename T, int L, bool DEV, bool VEC>
 __global__ __launch_bounds__(256) void k_sweep_x_wave(SweepArgs<T> a) {
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int nx = a.nx;
     const int P = (nx + L - 1) / L;
     const int RW = 64 / P;
     const int lane = threadIdx.x & 63;
@@ -353,95 +446,17 @@ __global__ __launch_bounds__(256) void k_sweep_x_wave(SweepArgs<T> a) {
     const int rows = row0 + (int)a.count;
     const int ngroups = ((int)a.count + RW - 1) / RW;
     const int wstride = gridDim.x * (blockDim.x >> 6);
-    const T k = a.k[d];
-    const T bt = DEV ? a.beta[d] : T(0);
-    const bool dact = (a.active >> d) & 1ull;
-    // Nodes past the row end get 1/den = c' = 0 and value 0: the recurrences
-    // then carry zeros through them and need no per-node masking.
+    // 1/den and c' of this lane's nodes, zero past the row end
     T inv[L], cp[L];
 #pragma unroll
     for (int m = 0; m < L; ++m) {
         inv[m] = m < valid ? a.inv[d * nx + s + m] : T(0);
         cp[m] = m < valid ? a.cp[d * nx + s + m] : T(0);
     }
-    T* ub = a.u + (long long)d * a.N;
     for (int g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); g < ngroups; g += wstride) {
         const int r = row0 + g * RW + rw;
         const bool have = lane_on && r < rows;
-        T cur[L];
-        if (have)
-            load_chunk<T, L, VEC>(cur, ub + r * nx + s, valid);
-        else {
-#pragma unroll
-            for (int m = 0; m < L; ++m) cur[m] = T(0);
-        }
-        // Dirichlet values of this row: both ends, and the interior when the
-        // whole row lies on a y or z face.
-        T v_lo = T(0), v_hi = T(0), v_in = T(0);
-        bool rowb = false;
-        if (have && dact) {
-            const int kk = r / ny;
-            const int jj = r - kk * ny;
-            rowb = (jj == 0 || jj == ny - 1 || kk == 0 || kk == nz - 1);
-            v_lo = a.dval[dir_slot(0, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
-            v_hi = a.dval[dir_slot(nx - 1, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
-            v_in = a.dval[dir_slot(-1, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
-        }
-        const bool dir_on = have && dact;
-        if (dir_on) dirichlet_row<T, L>(cur, s, nx, rowb, v_lo, v_hi, v_in);
-        if (DEV) {
-#pragma unroll
-            for (int m = 0; m < L; ++m) cur[m] = (m < valid) ? cur[m] - bt : T(0);
-        }
-        // forward: local zero-carry pass
-        T za = T(1), zb = T(0);
-#pragma unroll
-        for (int m = 0; m < L; ++m) {
-            zb = fma(k, zb, cur[m]) * inv[m];
-            za = za * (k * inv[m]);
-        }
-        // inclusive scan of y -> za*y + zb over the lanes of this row
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const T pa = shfl_up_t(za, off), pb = shfl_up_t(zb, off);
-            if (off < P && p >= off) {
-                zb = fma(za, pb, zb);
-                za = za * pa;
-            }
-        }
-        T y = shfl_up_t(zb, 1);
-        if (p == 0) y = T(0);
-#pragma unroll
-        for (int m = 0; m < L; ++m) {
-            y = fma(k, y, cur[m]) * inv[m];
-            cur[m] = y;
-        }
-        // backward: local zero-carry pass, then suffix scan
-        T wa = T(1), wb = T(0);
-#pragma unroll
-        for (int m = L - 1; m >= 0; --m) {
-            wb = fma(-cp[m], wb, cur[m]);
-            wa = wa * (-cp[m]);
-        }
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const T na = shfl_down_t(wa, off), nb = shfl_down_t(wb, off);
-            if (off < P && p + off < P) {
-                wb = fma(wa, nb, wb);
-                wa = wa * na;
-            }
-        }
-        T xn = shfl_down_t(wb, 1);
-        if (p == P - 1) xn = T(0);
-#pragma unroll
-        for (int m = L - 1; m >= 0; --m) {
-            xn = fma(-cp[m], xn, cur[m]);
-            cur[m] = xn;
-        }
-        if (have) {
-            if (dir_on) dirichlet_row<T, L>(cur, s, nx, rowb, v_lo, v_hi, v_in);
-            store_chunk<T, L, VEC>(cur, ub + r * nx + s, valid);
-        }
+        x_row_group<T, L, DEV, VEC>(a, d, r, have, p, P, s, valid, inv, cp);
     }
 }
 

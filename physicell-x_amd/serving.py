@@ -165,7 +165,9 @@ class Server:
         self._admit()
         if not self.active:
             return []
-        k = min(self.k, max(j.remaining for j in self.active.values()))
+        # a fixed run length keeps one captured graph; jobs needing fewer
+        # steps stop early inside it
+        k = self.k
         for mid, job in self.active.items():
             self.batch.set_member_steps(mid, min(k, job.remaining))
         self.batch.step(self.dt, k)
