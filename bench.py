@@ -386,6 +386,8 @@ def main():
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--mode", default="default", choices=["default", "single", "ensemble", "serve", "large"])
     ap.add_argument("--requests", type=int, default=40)
+    ap.add_argument("--tiles", default="", help="fixed FP32 shapes Lx,Ly,Lz,rows_x,cols_y,cols_z[,slab] "
+                    "instead of autotuning (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -407,7 +409,11 @@ def main():
     updates = n ** 3 * ns
 
     tiles = {"fp32": None, "fp64": None}
-    if not args.no_tune:
+    if args.tiles:
+        v = [int(x) for x in args.tiles.split(",")]
+        tiles["fp32"] = dict(Lx=v[0], Ly=v[1], Lz=v[2], rows_x=v[3], cols_y=v[4], cols_z=v[5],
+                             slab=v[6] if len(v) > 6 else 0)
+    elif not args.no_tune:
         import importlib
         at = importlib.import_module("physicellx_amd.autotune")
         for prec in ("fp32",) if args.no_fp64 else ("fp32", "fp64"):
