@@ -386,6 +386,7 @@ def main():
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--mode", default="default", choices=["default", "single", "ensemble", "serve", "large"])
     ap.add_argument("--requests", type=int, default=40)
+    ap.add_argument("--no-seq", action="store_true", help="ensemble mode: skip the sequential plain solves")
     ap.add_argument("--tiles", default="", help="fixed FP32 shapes Lx,Ly,Lz,rows_x,cols_y,cols_z[,slab] "
                     "instead of autotuning (profiling runs)")
     args = ap.parse_args()
@@ -395,7 +396,7 @@ def main():
         sys.exit(2)
     pkg = load_pkg()
     if args.mode == "ensemble":
-        print(json.dumps({"ensemble": run_ensemble(pkg)}))
+        print(json.dumps({"ensemble": run_ensemble(pkg, sequential=not args.no_seq)}))
         return
     if args.mode == "serve":
         print(json.dumps({"serving": run_serving(pkg, n_req=args.requests)}))

@@ -39,6 +39,8 @@ struct bfa_env {
     size_t tsize = 8;
     int device = 0;
     int max_lds = 65536;
+    int cus = 256;  // compute units of the device
+    int xw_occ = 1;  // resident blocks per CU of the chosen wave x-sweep kernel
     hipStream_t stream = nullptr;
     void* u = nullptr;
 
@@ -159,6 +161,16 @@ size_t lds_x(const bfa_env* e, int R, int L) {
     const int P = (e->nx + L - 1) / L;
     const int S = e->nx | 1;
     return ((size_t)R * S + 2 * (size_t)P * L + (size_t)P * R + P) * e->tsize;
+}
+
+// Resident blocks per CU of a kernel at the given block size (at least 1).
+static int occupancy_blocks(const void* fn, int block) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, 0) != hipSuccess || nb < 1) {
+        (void)hipGetLastError();
+        nb = 1;
+    }
+    return nb;
 }
 
 int choose_tiles(bfa_env* e) {
@@ -313,9 +325,11 @@ int launch_xw_L(bfa_env* e, hipStream_t st, const Win& w) {
     const int P = (nx + L - 1) / L;
     const int RW = 64 / P;
     const long long groups = (w.count + RW - 1) / RW;
-    const long long want = std::max<long long>(1, 1024 / w.nd);
-    const unsigned bx = (unsigned)std::max<long long>(1, std::min<long long>((groups + 3) / 4, want));
     const bool vec = ((size_t)nx * sizeof(T)) % 16 == 0 && ((size_t)L * sizeof(T)) % 16 == 0;
+    // one resident wave of blocks fills the device; each wavefront then
+    // strides over its row groups with the next one's loads in flight
+    const long long want = std::max<long long>(1, (long long)e->cus * e->xw_occ / w.nd);
+    const unsigned bx = (unsigned)std::max<long long>(1, std::min<long long>((groups + 3) / 4, want));
     bfa::SweepArgs<T> a = sweep_args<T>(e, 0);
     a.d0 = w.d0;
     a.first = w.first;
@@ -377,6 +391,17 @@ int raise_lds_limits(bfa_env* e) {
         if (need[a] > 65536 && fn[a])
             HIP_TRY(e, hipFuncSetAttribute(fn[a], hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)need[a]));
+    if (e->R == 0) {
+        const bool vec = ((size_t)e->nx * sizeof(T)) % 16 == 0 && ((size_t)e->L[0] * sizeof(T)) % 16 == 0;
+        const void* fw = nullptr;
+#define BFA_XW(Lv)                                                                   \
+    if (e->L[0] == Lv)                                                               \
+        fw = vec ? (const void*)bfa::k_sweep_x_wave<T, Lv, DEV, true>                 \
+                 : (const void*)bfa::k_sweep_x_wave<T, Lv, DEV, false>;
+        BFA_XW(4) BFA_XW(8) BFA_XW(16) BFA_XW(32)
+#undef BFA_XW
+        e->xw_occ = fw ? occupancy_blocks(fw, 256) : 1;
+    }
     return BFA_OK;
 }
 
@@ -657,6 +682,10 @@ int bfa_create(bfa_env** out, int nx, int ny, int nz, int ns, double dx, int pre
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, e->device) == hipSuccess &&
         lds > 0)
         e->max_lds = lds;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device) == hipSuccess &&
+        cus > 0)
+        e->cus = cus;
     st = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
     if (st != hipSuccess) {
         fail(e, BFA_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(st));

@@ -40,9 +40,10 @@ struct EnvDesc {
 struct XItem {
     int env, d, row0, nrows;
 };
-// y / z sweep item: columns [c0, c0 + TX) of substrate d of member env.
+// y / z sweep item: columns [c0, c0 + ntiles*TX) of substrate d of member
+// env, solved as ntiles tiles of TX columns.
 struct ColItem {
-    int env, d, c0, pad;
+    int env, d, c0, ntiles;
 };
 
 template <typename T>
@@ -68,7 +69,9 @@ __device__ __forceinline__ SweepArgs<T> args_of(const EnvDesc<T>& e, int axis) {
 }
 
 // Persistent batched x sweep.  Each wavefront takes XItems from ticket[0]
-// until *nitems is reached.  Members need nx <= 64*L.
+// until *nitems is reached; the ticket of its next item is drawn while the
+// current one runs, and within an item the loads of the next row group are in
+// flight while the current one is solved.  Members need nx <= 64*L.
 template <typename T, int L, bool DEV>
 __global__ __launch_bounds__(256) void kb_sweep_x(const EnvDesc<T>* __restrict__ envs,
                                                   const XItem* __restrict__ items,
@@ -76,11 +79,12 @@ __global__ __launch_bounds__(256) void kb_sweep_x(const EnvDesc<T>* __restrict__
                                                   int* __restrict__ ticket, int step) {
     const int lane = threadIdx.x & 63;
     const int n_items = *nitems;
+    int nxt = 0;
+    if (lane == 0) nxt = atomicAdd(ticket, 1);
     for (;;) {
-        int it = 0;
-        if (lane == 0) it = atomicAdd(ticket, 1);
-        it = __shfl(it, 0, 64);
+        const int it = __shfl(nxt, 0, 64);
         if (it >= n_items) break;
+        if (lane == 0) nxt = atomicAdd(ticket, 1);
         const XItem w = items[it];
         if (step >= envs[w.env].steps) continue;
         const SweepArgs<T> a = args_of<T>(envs[w.env], 0);
@@ -93,28 +97,23 @@ __global__ __launch_bounds__(256) void kb_sweep_x(const EnvDesc<T>* __restrict__
         const bool lane_on = rw < RW;
         const int s = p * L;
         const int valid = lane_on ? (nx - s < L ? nx - s : L) : 0;
-        T inv[L], cp[L];
-#pragma unroll
-        for (int m = 0; m < L; ++m) {
-            inv[m] = m < valid ? a.inv[w.d * nx + s + m] : T(0);
-            cp[m] = m < valid ? a.cp[w.d * nx + s + m] : T(0);
-        }
+        XCoef<T, L> c;
+        x_coef_setup<T, L, DEV>(a, w.d, p, P, s, valid, c);
         const bool vec = ((nx * (int)sizeof(T)) & 15) == 0 && ((L * (int)sizeof(T)) & 15) == 0;
         const int rend = w.row0 + w.nrows;
-        for (int r0 = w.row0; r0 < rend; r0 += RW) {
-            const int r = r0 + rw;
-            const bool have = lane_on && r < rend;
-            if (vec)
-                x_row_group<T, L, DEV, true>(a, w.d, r, have, p, P, s, valid, inv, cp);
-            else
-                x_row_group<T, L, DEV, false>(a, w.d, r, have, p, P, s, valid, inv, cp);
-        }
+        if (vec)
+            x_rows_pipelined<T, L, DEV, true>(a, w.d, w.row0, rend, RW, rw, lane_on, p, P, s, valid, c);
+        else
+            x_rows_pipelined<T, L, DEV, false>(a, w.d, w.row0, rend, RW, rw, lane_on, p, P, s, valid, c);
     }
 }
 
 // Persistent batched y (AXIS 1) or z (AXIS 2) sweep.  Each block takes
 // ColItems from ticket[0]; blockDim.x = TX * PMAX with PMAX >= ceil(n/L) for
-// every member.  Dynamic LDS: (2*PMAX*L + TX*PMAX + PMAX) * sizeof(T) + 16.
+// every member.  The ticket of the next item is drawn while the current one
+// runs (two LDS slots, alternating), and the column tiles of an item are
+// pipelined: the loads of tile q+1 are issued before tile q is solved.
+// Dynamic LDS: (2*PMAX*L + TX*PMAX + PMAX) * sizeof(T) + 16.
 template <typename T, int L, int AXIS, bool DEV, int TX>
 __global__ __launch_bounds__(1024) void kb_sweep_col(const EnvDesc<T>* __restrict__ envs,
                                                      const ColItem* __restrict__ items,
@@ -131,14 +130,18 @@ __global__ __launch_bounds__(1024) void kb_sweep_col(const EnvDesc<T>* __restric
     const int lane = t % TX;
     const int p = t / TX;
     const int n_items = *nitems;
-    for (;;) {
-        __syncthreads();  // previous item's LDS readers are done
-        if (t == 0) *s_item = atomicAdd(ticket, 1);
-        __syncthreads();
-        const int it = *s_item;
+    if (t == 0) s_item[0] = atomicAdd(ticket, 1);
+    for (int slot = 0;; slot ^= 1) {
+        __syncthreads();  // s_item[slot] written; the previous item's LDS readers are done
+        const int it = s_item[slot];
         if (it >= n_items) break;
+        int nxt = 0;
+        if (t == 0) nxt = atomicAdd(ticket, 1);
         const ColItem w = items[it];
-        if (step >= envs[w.env].steps) continue;  // uniform over the block
+        if (step >= envs[w.env].steps) {  // uniform over the block
+            if (t == 0) s_item[slot ^ 1] = nxt;
+            continue;
+        }
         const SweepArgs<T> a = args_of<T>(envs[w.env], AXIS);
         const int nx = a.nx, ny = a.ny, nz = a.nz;
         const int n = AXIS == 1 ? ny : nz;
@@ -147,49 +150,62 @@ __global__ __launch_bounds__(1024) void kb_sweep_col(const EnvDesc<T>* __restric
         stage_coefs<T>(s_inv, s_cp, a.inv + (long long)d * n, a.cp + (long long)d * n, n, P * L, t,
                        blockDim.x);
         const long long ncols = AXIS == 1 ? (long long)nz * nx : (long long)ny * nx;
-        const long long c = (long long)w.c0 + lane;
-        const bool act = p < P && c < ncols;
-        long long base = 0, stride;
-        int ci = 0, co = 0;
-        if (act) {
+        const long long stride = AXIS == 1 ? (long long)nx : (long long)nx * ny;
+        const int s = p * L;
+        const T bt = DEV ? a.beta[d] : T(0);
+        const T kd = a.k[d];
+        const bool dact = (a.active >> d) & 1ull;
+        // base offset of column c, or -1 when the lane has no column
+        auto col_base = [&](long long c, int& ci, int& co) -> long long {
+            if (p >= P || c >= ncols) return -1;
             co = (int)(c / nx);
             ci = (int)(c - (long long)co * nx);
-        }
-        if (AXIS == 1) {
-            base = (long long)d * a.N + (long long)co * ny * nx + ci;
-            stride = nx;
-        } else {
-            base = (long long)d * a.N + c;
-            stride = (long long)nx * ny;
-        }
-        const int s = p * L;
-        T x[L];
+            return AXIS == 1 ? (long long)d * a.N + (long long)co * ny * nx + ci
+                             : (long long)d * a.N + c;
+        };
+        T xn[L];
+        int ci_n = 0, co_n = 0;
+        long long base_n = col_base((long long)w.c0 + lane, ci_n, co_n);
 #pragma unroll
-        for (int m = 0; m < L; ++m) x[m] = (act && s + m < n) ? a.u[base + (s + m) * stride] : T(0);
-        if (DEV) {
-            const T bt = a.beta[d];
+        for (int m = 0; m < L; ++m)
+            xn[m] = (base_n >= 0 && s + m < n) ? a.u[base_n + (s + m) * stride] : T(0);
+        if (t == 0) s_item[slot ^ 1] = nxt;
+        for (int q = 0; q < w.ntiles; ++q) {
+            T x[L];
 #pragma unroll
-            for (int m = 0; m < L; ++m) x[m] = (s + m < n) ? x[m] - bt : T(0);
-        }
-        __syncthreads();  // staging done, s_item read by all
-        chunk_solve<T, L>(x, act, p, P, lane, TX, s_inv, s_cp, a.k[d], s_z, s_g);
-        const bool dact = (a.active >> d) & 1ull;
-        if (act && dact) {
-            const bool lineb = AXIS == 1 ? (ci == 0 || ci == nx - 1 || co == 0 || co == nz - 1)
-                                         : (ci == 0 || ci == nx - 1 || co == 0 || co == ny - 1);
-            const int s_lo = AXIS == 1 ? dir_slot(ci, 0, co, nx, ny, nz, a.reg)
-                                       : dir_slot(ci, co, 0, nx, ny, nz, a.reg);
-            const int s_hi = AXIS == 1 ? dir_slot(ci, n - 1, co, nx, ny, nz, a.reg)
-                                       : dir_slot(ci, co, n - 1, nx, ny, nz, a.reg);
-            const int s_in = AXIS == 1 ? dir_slot(ci, -1, co, nx, ny, nz, a.reg)
-                                       : dir_slot(ci, co, -1, nx, ny, nz, a.reg);
-            dirichlet_row<T, L>(x, s, n, lineb, a.dval[s_lo * a.ns + d], a.dval[s_hi * a.ns + d],
-                                a.dval[s_in * a.ns + d]);
-        }
-        if (act) {
+            for (int m = 0; m < L; ++m) x[m] = xn[m];
+            const long long base = base_n;
+            const int ci = ci_n, co = co_n;
+            if (q + 1 < w.ntiles) {
+                base_n = col_base((long long)w.c0 + (long long)(q + 1) * TX + lane, ci_n, co_n);
 #pragma unroll
-            for (int m = 0; m < L; ++m)
-                if (s + m < n) a.u[base + (s + m) * stride] = x[m];
+                for (int m = 0; m < L; ++m)
+                    xn[m] = (base_n >= 0 && s + m < n) ? a.u[base_n + (s + m) * stride] : T(0);
+            }
+            const bool act = base >= 0;
+            if (DEV) {
+#pragma unroll
+                for (int m = 0; m < L; ++m) x[m] = (s + m < n) ? x[m] - bt : T(0);
+            }
+            __syncthreads();  // staging done / the previous tile's fold readers are done
+            chunk_solve<T, L>(x, act, p, P, lane, TX, s_inv, s_cp, kd, s_z, s_g);
+            if (act && dact) {
+                const bool lineb = AXIS == 1 ? (ci == 0 || ci == nx - 1 || co == 0 || co == nz - 1)
+                                             : (ci == 0 || ci == nx - 1 || co == 0 || co == ny - 1);
+                const int s_lo = AXIS == 1 ? dir_slot(ci, 0, co, nx, ny, nz, a.reg)
+                                           : dir_slot(ci, co, 0, nx, ny, nz, a.reg);
+                const int s_hi = AXIS == 1 ? dir_slot(ci, n - 1, co, nx, ny, nz, a.reg)
+                                           : dir_slot(ci, co, n - 1, nx, ny, nz, a.reg);
+                const int s_in = AXIS == 1 ? dir_slot(ci, -1, co, nx, ny, nz, a.reg)
+                                           : dir_slot(ci, co, -1, nx, ny, nz, a.reg);
+                dirichlet_row<T, L>(x, s, n, lineb, a.dval[s_lo * a.ns + d],
+                                    a.dval[s_hi * a.ns + d], a.dval[s_in * a.ns + d]);
+            }
+            if (act) {
+#pragma unroll
+                for (int m = 0; m < L; ++m)
+                    if (s + m < n) a.u[base + (s + m) * stride] = x[m];
+            }
         }
     }
 }

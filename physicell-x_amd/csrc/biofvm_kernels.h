@@ -339,94 +339,206 @@ __device__ __forceinline__ T shfl_up_t(T v, int d) { return __shfl_up(v, d, 64);
 template <typename T>
 __device__ __forceinline__ T shfl_down_t(T v, int d) { return __shfl_down(v, d, 64); }
 
-// One row group of the wave-local x sweep: rows r..r+RW share the
-// wavefront, lane p of each row owns nodes [p*L, p*L+L) of row r (its
-// 1/den and c' in inv/cp, zero past the row end).  Dirichlet passes D0 and
-// D1 are applied to the row's end nodes, and to all its nodes when the row
-// lies on a y or z face.  Used by the single-microenvironment and the batch
-// x sweeps.
-template <typename T, int L, bool DEV, bool VEC>
-__device__ __forceinline__ void x_row_group(const SweepArgs<T>& a, int d, int r, bool have,
-                                            int p, int P, int s, int valid, const T (&inv)[L],
-                                            const T (&cp)[L]) {
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
-    T* ub = a.u + (long long)d * a.N;
-    const T k = a.k[d];
-    const bool dact = (a.active >> d) & 1ull;
-    T cur[L];
-    if (have)
-        load_chunk<T, L, VEC>(cur, ub + (long long)r * nx + s, valid);
-    else {
-#pragma unroll
-        for (int m = 0; m < L; ++m) cur[m] = T(0);
-    }
-    // lanes holding a row end, or every lane of a face row, need Dirichlet
-    // values; the others skip the lookups
-    const bool has_end = s == 0 || s + valid == nx;
-    T v_lo = T(0), v_hi = T(0), v_in = T(0);
-    bool rowb = false, dir_on = false;
-    if (have && dact) {
-        const int kk = r / ny;
-        const int jj = r - kk * ny;
-        rowb = (jj == 0 || jj == ny - 1 || kk == 0 || kk == nz - 1);
-        dir_on = rowb || has_end;
-        if (dir_on) {
-            v_lo = a.dval[dir_slot(0, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
-            v_hi = a.dval[dir_slot(nx - 1, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
-            if (rowb) v_in = a.dval[dir_slot(-1, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
-            dirichlet_row<T, L>(cur, s, nx, rowb, v_lo, v_hi, v_in);
-        }
-    }
-    if (DEV) {
-        const T bt = a.beta[d];
-#pragma unroll
-        for (int m = 0; m < L; ++m) cur[m] = (m < valid) ? cur[m] - bt : T(0);
-    }
-    T za = T(1), zb = T(0);
+// The workgroups of a launch are dispatched round-robin over the 8 XCDs
+// (linear workgroup id mod 8), each XCD with its own L2.  xcd_block maps the
+// linear id b of G workgroups to a logical tile index such that every XCD
+// runs one contiguous run of tiles, in order: neighbouring tiles that share
+// cache lines at their edges then meet in the same L2.  A bijection on [0, G).
+__device__ __forceinline__ int xcd_block(int b, int G) {
+    const int q = G >> 3, r = G & 7;
+    const int x = b & 7, i = b >> 3;
+    return x * q + (x < r ? x : r) + i;
+}
+
+// ---- wave-local x sweep: per-lane state and row routines ------------------
+// A row's P = ceil(nx/L) chunks sit in P consecutive lanes of a wavefront
+// (RW = 64/P rows per wavefront); lane p owns nodes [p*L, p*L+L).  The chunk
+// carries are combined by a Hillis-Steele scan of the affine maps
+// y -> A*y + B over the lanes of the row.  The A parts (products of the
+// multipliers k/den and -c') depend on the coefficients only, so every
+// lane's scan multipliers are computed once (XCoef) and each row's scan
+// shuffles only the B parts.
+
+// Dirichlet face slot of an x-end node of a row that is on no y or z face.
+__device__ __forceinline__ int dir_slot_xonly(int i, int nx, unsigned reg) {
+    if (i == nx - 1 && (reg & 2u)) return 1;
+    if (i == 0 && (reg & 1u)) return 0;
+    return 6;
+}
+
+template <typename T, int L>
+struct XCoef {
+    T inv[L], cp[L];  // 1/den and c' of the lane's nodes, zero past the row end
+    T fm[6], bm[6];   // multiplier of forward / backward scan step j, 0 where the step is off
+    T k, bt;          // D*dt/dx^2 and the deviation shift
+    T v_lo, v_hi;     // Dirichlet values at the x ends of rows on no y/z face
+    bool dact;        // Dirichlet on for this substrate
+};
+
+template <typename T, int L, bool DEV>
+__device__ __forceinline__ void x_coef_setup(const SweepArgs<T>& a, int d, int p, int P, int s,
+                                             int valid, XCoef<T, L>& c) {
+    const int nx = a.nx;
 #pragma unroll
     for (int m = 0; m < L; ++m) {
-        zb = fma(k, zb, cur[m]) * inv[m];
-        za = za * (k * inv[m]);
+        c.inv[m] = m < valid ? a.inv[d * nx + s + m] : T(0);
+        c.cp[m] = m < valid ? a.cp[d * nx + s + m] : T(0);
     }
+    c.k = a.k[d];
+    c.bt = DEV ? a.beta[d] : T(0);
+    c.dact = (a.active >> d) & 1ull;
+    c.v_lo = a.dval[dir_slot_xonly(0, nx, a.reg) * a.ns + d];
+    c.v_hi = a.dval[dir_slot_xonly(nx - 1, nx, a.reg) * a.ns + d];
+    T za = T(1);
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const T pa = shfl_up_t(za, off), pb = shfl_up_t(zb, off);
-        if (off < P && p >= off) {
-            zb = fma(za, pb, zb);
-            za = za * pa;
+    for (int m = 0; m < L; ++m) za = za * (c.k * c.inv[m]);
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        const int off = 1 << j;
+        const T pa = shfl_up_t(za, off);
+        const bool on = off < P && p >= off;
+        c.fm[j] = on ? za : T(0);
+        if (on) za = za * pa;
+    }
+    T wa = T(1);
+#pragma unroll
+    for (int m = L - 1; m >= 0; --m) wa = wa * (-c.cp[m]);
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        const int off = 1 << j;
+        const T na = shfl_down_t(wa, off);
+        const bool on = off < P && p + off < P;
+        c.bm[j] = on ? wa : T(0);
+        if (on) wa = wa * na;
+    }
+}
+
+// One row group.  x_row_fetch issues the row's loads (the field chunk, and
+// the per-row Dirichlet values of a row on a y or z face); x_row_solve
+// consumes them, so a caller can fetch the next row group before solving the
+// current one.  Dirichlet passes D0 (before) and D1 (after the solve) set
+// the row's end nodes, and all its nodes when the row lies on a y or z face.
+template <typename T, int L>
+struct XRow {
+    T v[L];
+    T f_lo, f_hi, f_in;  // Dirichlet values of a row on a y or z face
+    bool rowb;           // the row is on a y or z face and Dirichlet is on
+};
+
+template <typename T, int L, bool VEC>
+__device__ __forceinline__ void x_row_fetch(const SweepArgs<T>& a, int d, int r, bool have, int s,
+                                            int valid, const XCoef<T, L>& c, XRow<T, L>& q) {
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const T* ub = a.u + (long long)d * a.N;
+    if (have)
+        load_chunk<T, L, VEC>(q.v, ub + (long long)r * nx + s, valid);
+    else {
+#pragma unroll
+        for (int m = 0; m < L; ++m) q.v[m] = T(0);
+    }
+    q.f_lo = q.f_hi = q.f_in = T(0);
+    q.rowb = false;
+    if (have && c.dact) {
+        const int kk = r / ny;
+        const int jj = r - kk * ny;
+        q.rowb = (jj == 0 || jj == ny - 1 || kk == 0 || kk == nz - 1);
+        if (q.rowb) {
+            q.f_lo = a.dval[dir_slot(0, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
+            q.f_hi = a.dval[dir_slot(nx - 1, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
+            q.f_in = a.dval[dir_slot(-1, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
+        }
+    }
+}
+
+// Dirichlet pass of one row chunk (see x_row_fetch).
+template <typename T, int L>
+__device__ __forceinline__ void x_row_dirichlet(T (&cur)[L], int nx, int s, bool have,
+                                                const XCoef<T, L>& c, const XRow<T, L>& q) {
+    if (q.rowb) {
+        dirichlet_row<T, L>(cur, s, nx, true, q.f_lo, q.f_hi, q.f_in);
+    } else if (have && c.dact) {
+        if (s == 0) cur[0] = c.v_lo;
+        if (nx % L == 0) {
+            if (s + L == nx) cur[L - 1] = c.v_hi;
+        } else {
+#pragma unroll
+            for (int m = 0; m < L; ++m)
+                if (s + m == nx - 1) cur[m] = c.v_hi;
+        }
+    }
+}
+
+template <typename T, int L, bool DEV, bool VEC>
+__device__ __forceinline__ void x_row_solve(const SweepArgs<T>& a, int d, int r, bool have, int p,
+                                            int P, int s, int valid, const XCoef<T, L>& c,
+                                            XRow<T, L>& q) {
+    const int nx = a.nx;
+    T(&cur)[L] = q.v;
+    x_row_dirichlet<T, L>(cur, nx, s, have, c, q);
+    // past the row end 1/den = c' = 0, so whatever sits there leaves the
+    // row's values unchanged and becomes 0 in the forward pass
+    if (DEV) {
+#pragma unroll
+        for (int m = 0; m < L; ++m) cur[m] = cur[m] - c.bt;
+    }
+    T zb = T(0);
+#pragma unroll
+    for (int m = 0; m < L; ++m) zb = fma(c.k, zb, cur[m]) * c.inv[m];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        const int off = 1 << j;
+        if (off < P) {
+            const T pb = shfl_up_t(zb, off);
+            zb = fma(c.fm[j], pb, zb);
         }
     }
     T y = shfl_up_t(zb, 1);
     if (p == 0) y = T(0);
 #pragma unroll
     for (int m = 0; m < L; ++m) {
-        y = fma(k, y, cur[m]) * inv[m];
+        y = fma(c.k, y, cur[m]) * c.inv[m];
         cur[m] = y;
     }
-    T wa = T(1), wb = T(0);
+    T wb = T(0);
 #pragma unroll
-    for (int m = L - 1; m >= 0; --m) {
-        wb = fma(-cp[m], wb, cur[m]);
-        wa = wa * (-cp[m]);
-    }
+    for (int m = L - 1; m >= 0; --m) wb = fma(-c.cp[m], wb, cur[m]);
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const T na = shfl_down_t(wa, off), nb = shfl_down_t(wb, off);
-        if (off < P && p + off < P) {
-            wb = fma(wa, nb, wb);
-            wa = wa * na;
+    for (int j = 0; j < 6; ++j) {
+        const int off = 1 << j;
+        if (off < P) {
+            const T nb = shfl_down_t(wb, off);
+            wb = fma(c.bm[j], nb, wb);
         }
     }
     T xn = shfl_down_t(wb, 1);
     if (p == P - 1) xn = T(0);
 #pragma unroll
     for (int m = L - 1; m >= 0; --m) {
-        xn = fma(-cp[m], xn, cur[m]);
+        xn = fma(-c.cp[m], xn, cur[m]);
         cur[m] = xn;
     }
     if (have) {
-        if (dir_on) dirichlet_row<T, L>(cur, s, nx, rowb, v_lo, v_hi, v_in);
-        store_chunk<T, L, VEC>(cur, ub + (long long)r * nx + s, valid);
+        x_row_dirichlet<T, L>(cur, nx, s, have, c, q);
+        store_chunk<T, L, VEC>(cur, a.u + (long long)d * a.N + (long long)r * nx + s, valid);
+    }
+}
+
+// Rows [r0, r1) of substrate d in row groups of RW, pipelined: the next
+// group's loads are issued before the current group is solved.
+template <typename T, int L, bool DEV, bool VEC>
+__device__ __forceinline__ void x_rows_pipelined(const SweepArgs<T>& a, int d, int r0, int r1,
+                                                 int RW, int rw, bool lane_on, int p, int P, int s,
+                                                 int valid, const XCoef<T, L>& c) {
+    XRow<T, L> nxt;
+    x_row_fetch<T, L, VEC>(a, d, r0 + rw, lane_on && r0 + rw < r1, s, valid, c, nxt);
+    for (int g = r0; g < r1; g += RW) {
+        XRow<T, L> cur = nxt;
+        const int r = g + rw;
+        if (g + RW < r1) {
+            const int rn = r + RW;
+            x_row_fetch<T, L, VEC>(a, d, rn, lane_on && rn < r1, s, valid, c, nxt);
+        }
+        x_row_solve<T, L, DEV, VEC>(a, d, r, lane_on && r < r1, p, P, s, valid, c, cur);
     }
 }
 
@@ -439,24 +551,34 @@ __global__ __launch_bounds__(256) void k_sweep_x_wave(SweepArgs<T> a) {
     const int rw = lane / P;
     const int p = lane - rw * P;
     const bool lane_on = rw < RW;
-    const int d = a.d0 + blockIdx.y;
+    const int lb = xcd_block(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int bx = lb % gridDim.x;
+    const int d = a.d0 + lb / gridDim.x;
     const int s = p * L;
     const int valid = lane_on ? (nx - s < L ? nx - s : L) : 0;
     const int row0 = (int)a.first;
     const int rows = row0 + (int)a.count;
     const int ngroups = ((int)a.count + RW - 1) / RW;
     const int wstride = gridDim.x * (blockDim.x >> 6);
-    // 1/den and c' of this lane's nodes, zero past the row end
-    T inv[L], cp[L];
-#pragma unroll
-    for (int m = 0; m < L; ++m) {
-        inv[m] = m < valid ? a.inv[d * nx + s + m] : T(0);
-        cp[m] = m < valid ? a.cp[d * nx + s + m] : T(0);
-    }
-    for (int g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); g < ngroups; g += wstride) {
+    XCoef<T, L> c;
+    x_coef_setup<T, L, DEV>(a, d, p, P, s, valid, c);
+    // two-stage pipeline: the loads of the next row group are in flight
+    // while the current one is solved
+    int g = bx * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    XRow<T, L> nxt;
+    {
         const int r = row0 + g * RW + rw;
-        const bool have = lane_on && r < rows;
-        x_row_group<T, L, DEV, VEC>(a, d, r, have, p, P, s, valid, inv, cp);
+        x_row_fetch<T, L, VEC>(a, d, r, g < ngroups && lane_on && r < rows, s, valid, c, nxt);
+    }
+    for (; g < ngroups; g += wstride) {
+        XRow<T, L> cur = nxt;
+        const int r = row0 + g * RW + rw;
+        const int gn = g + wstride;
+        if (gn < ngroups) {
+            const int rn = row0 + gn * RW + rw;
+            x_row_fetch<T, L, VEC>(a, d, rn, lane_on && rn < rows, s, valid, c, nxt);
+        }
+        x_row_solve<T, L, DEV, VEC>(a, d, r, lane_on && r < rows, p, P, s, valid, c, cur);
     }
 }
 
@@ -478,10 +600,11 @@ __global__ __launch_bounds__(1024) void k_sweep_col(SweepArgs<T> a) {
     const int t = threadIdx.x;
     const int lane = t % TX;
     const int p = t / TX;
-    const int d = a.d0 + blockIdx.y;
+    const int lb = xcd_block(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int d = a.d0 + lb / gridDim.x;
     stage_coefs<T>(s_inv, s_cp, a.inv + (long long)d * n, a.cp + (long long)d * n, n, P * L, t,
                    blockDim.x);
-    const long long c = a.first + (long long)blockIdx.x * TX + lane;
+    const long long c = a.first + (long long)(lb % gridDim.x) * TX + lane;
     const bool act = c < a.first + a.count;
     long long base = 0, stride;
     int ci = 0, co = 0;  // i and the other transverse coordinate (k for y, j for z)
