@@ -82,7 +82,7 @@ __global__ __launch_bounds__(256) void kb_sweep_x(const EnvDesc<T>* __restrict__
     int nxt = 0;
     if (lane == 0) nxt = atomicAdd(ticket, 1);
     for (;;) {
-        const int it = __shfl(nxt, 0, 64);
+        const int it = __builtin_amdgcn_readfirstlane(nxt);  // lane 0's ticket
         if (it >= n_items) break;
         if (lane == 0) nxt = atomicAdd(ticket, 1);
         const XItem w = items[it];
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(1024) void kb_sweep_col(const EnvDesc<T>* __restric
     if (t == 0) s_item[0] = atomicAdd(ticket, 1);
     for (int slot = 0;; slot ^= 1) {
         __syncthreads();  // s_item[slot] written; the previous item's LDS readers are done
-        const int it = s_item[slot];
+        const int it = __builtin_amdgcn_readfirstlane(s_item[slot]);
         if (it >= n_items) break;
         int nxt = 0;
         if (t == 0) nxt = atomicAdd(ticket, 1);
@@ -149,63 +149,49 @@ __global__ __launch_bounds__(1024) void kb_sweep_col(const EnvDesc<T>* __restric
         const int d = w.d;
         stage_coefs<T>(s_inv, s_cp, a.inv + (long long)d * n, a.cp + (long long)d * n, n, P * L, t,
                        blockDim.x);
-        const long long ncols = AXIS == 1 ? (long long)nz * nx : (long long)ny * nx;
-        const long long stride = AXIS == 1 ? (long long)nx : (long long)nx * ny;
+        const int ncols = AXIS == 1 ? nz * nx : ny * nx;
+        const size_t stride = AXIS == 1 ? (size_t)nx : (size_t)nx * ny;
+        T* ub = a.u + (long long)d * a.N;
         const int s = p * L;
         const T bt = DEV ? a.beta[d] : T(0);
         const T kd = a.k[d];
         const bool dact = (a.active >> d) & 1ull;
-        // base offset of column c, or -1 when the lane has no column
-        auto col_base = [&](long long c, int& ci, int& co) -> long long {
-            if (p >= P || c >= ncols) return -1;
-            co = (int)(c / nx);
-            ci = (int)(c - (long long)co * nx);
-            return AXIS == 1 ? (long long)d * a.N + (long long)co * ny * nx + ci
-                             : (long long)d * a.N + c;
+        // column c of the item: its (ci, co), offset of its first node, and
+        // whether this thread holds part of it
+        auto col_of = [&](int c, int& ci, int& co, unsigned& base) -> bool {
+            const bool on = p < P && c < ncols;
+            co = on ? c / nx : 0;
+            ci = on ? c - co * nx : 0;
+            base = AXIS == 1 ? (unsigned)(co * ny * nx + ci) : (unsigned)c;
+            return on;
         };
         T xn[L];
-        int ci_n = 0, co_n = 0;
-        long long base_n = col_base((long long)w.c0 + lane, ci_n, co_n);
-#pragma unroll
-        for (int m = 0; m < L; ++m)
-            xn[m] = (base_n >= 0 && s + m < n) ? a.u[base_n + (s + m) * stride] : T(0);
+        int ci_n, co_n;
+        unsigned base_n;
+        bool act_n = col_of(w.c0 + lane, ci_n, co_n, base_n);
+        load_line_chunk<T, L>(xn, ub + (size_t)base_n + (size_t)s * stride, stride, act_n ? n - s : 0);
         if (t == 0) s_item[slot ^ 1] = nxt;
         for (int q = 0; q < w.ntiles; ++q) {
             T x[L];
 #pragma unroll
             for (int m = 0; m < L; ++m) x[m] = xn[m];
-            const long long base = base_n;
+            const unsigned base = base_n;
             const int ci = ci_n, co = co_n;
+            const bool act = act_n;
             if (q + 1 < w.ntiles) {
-                base_n = col_base((long long)w.c0 + (long long)(q + 1) * TX + lane, ci_n, co_n);
-#pragma unroll
-                for (int m = 0; m < L; ++m)
-                    xn[m] = (base_n >= 0 && s + m < n) ? a.u[base_n + (s + m) * stride] : T(0);
+                act_n = col_of(w.c0 + (q + 1) * TX + lane, ci_n, co_n, base_n);
+                load_line_chunk<T, L>(xn, ub + (size_t)base_n + (size_t)s * stride, stride,
+                                      act_n ? n - s : 0);
             }
-            const bool act = base >= 0;
             if (DEV) {
+                // padded nodes (past the line end) do not influence the line
 #pragma unroll
-                for (int m = 0; m < L; ++m) x[m] = (s + m < n) ? x[m] - bt : T(0);
+                for (int m = 0; m < L; ++m) x[m] = x[m] - bt;
             }
             __syncthreads();  // staging done / the previous tile's fold readers are done
             chunk_solve<T, L>(x, act, p, P, lane, TX, s_inv, s_cp, kd, s_z, s_g);
-            if (act && dact) {
-                const bool lineb = AXIS == 1 ? (ci == 0 || ci == nx - 1 || co == 0 || co == nz - 1)
-                                             : (ci == 0 || ci == nx - 1 || co == 0 || co == ny - 1);
-                const int s_lo = AXIS == 1 ? dir_slot(ci, 0, co, nx, ny, nz, a.reg)
-                                           : dir_slot(ci, co, 0, nx, ny, nz, a.reg);
-                const int s_hi = AXIS == 1 ? dir_slot(ci, n - 1, co, nx, ny, nz, a.reg)
-                                           : dir_slot(ci, co, n - 1, nx, ny, nz, a.reg);
-                const int s_in = AXIS == 1 ? dir_slot(ci, -1, co, nx, ny, nz, a.reg)
-                                           : dir_slot(ci, co, -1, nx, ny, nz, a.reg);
-                dirichlet_row<T, L>(x, s, n, lineb, a.dval[s_lo * a.ns + d],
-                                    a.dval[s_hi * a.ns + d], a.dval[s_in * a.ns + d]);
-            }
-            if (act) {
-#pragma unroll
-                for (int m = 0; m < L; ++m)
-                    if (s + m < n) a.u[base + (s + m) * stride] = x[m];
-            }
+            if (act && dact) col_dirichlet<T, L, AXIS>(x, a, d, ci, co, s, n, p, P);
+            store_line_chunk<T, L>(x, ub + (size_t)base + (size_t)s * stride, stride, act ? n - s : 0);
         }
     }
 }

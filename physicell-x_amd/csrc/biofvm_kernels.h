@@ -551,9 +551,11 @@ __global__ __launch_bounds__(256) void k_sweep_x_wave(SweepArgs<T> a) {
     const int rw = lane / P;
     const int p = lane - rw * P;
     const bool lane_on = rw < RW;
-    const int lb = xcd_block(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
-    const int bx = lb % gridDim.x;
-    const int d = a.d0 + lb / gridDim.x;
+    // uniform over the block: keep the tile and substrate in scalar registers
+    const int lb = __builtin_amdgcn_readfirstlane(
+        xcd_block(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y));
+    const int bx = __builtin_amdgcn_readfirstlane(lb % gridDim.x);
+    const int d = __builtin_amdgcn_readfirstlane(a.d0 + lb / gridDim.x);
     const int s = p * L;
     const int valid = lane_on ? (nx - s < L ? nx - s : L) : 0;
     const int row0 = (int)a.first;
@@ -584,8 +586,64 @@ __global__ __launch_bounds__(256) void k_sweep_x_wave(SweepArgs<T> a) {
 
 // ---- y / z sweeps: one lane per x-column, TX columns per block -------------
 // AXIS 1 = y (line stride nx, columns (k,i)), AXIS 2 = z (stride nx*ny,
-// columns (j,i)).  blockDim.x = TX*P with P = ceil(n/L);
-// grid = (ceil(ncols/TX), ns).  Dynamic LDS: (2*P*L + TX*P + P) * sizeof(T).
+// columns (j,i)).  Offsets within a substrate are 32-bit (N < 2^31).
+
+// Dirichlet pass after the solve of column (ci, co): the line's end nodes,
+// and all its nodes when the line lies on a face parallel to it.  Only the
+// threads holding an end node look values up, unless the line is on a face.
+template <typename T, int L, int AXIS>
+__device__ __forceinline__ void col_dirichlet(T (&x)[L], const SweepArgs<T>& a, int d, int ci,
+                                              int co, int s, int n, int p, int P) {
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const bool lineb = AXIS == 1 ? (ci == 0 || ci == nx - 1 || co == 0 || co == nz - 1)
+                                 : (ci == 0 || ci == nx - 1 || co == 0 || co == ny - 1);
+    const int s_lo = AXIS == 1 ? dir_slot(ci, 0, co, nx, ny, nz, a.reg)
+                               : dir_slot(ci, co, 0, nx, ny, nz, a.reg);
+    const int s_hi = AXIS == 1 ? dir_slot(ci, n - 1, co, nx, ny, nz, a.reg)
+                               : dir_slot(ci, co, n - 1, nx, ny, nz, a.reg);
+    if (lineb) {
+        const int s_in = AXIS == 1 ? dir_slot(ci, -1, co, nx, ny, nz, a.reg)
+                                   : dir_slot(ci, co, -1, nx, ny, nz, a.reg);
+        dirichlet_row<T, L>(x, s, n, true, a.dval[s_lo * a.ns + d], a.dval[s_hi * a.ns + d],
+                            a.dval[s_in * a.ns + d]);
+    } else {
+        if (p == 0) x[0] = a.dval[s_lo * a.ns + d];
+        if (p == P - 1) {
+            const T v = a.dval[s_hi * a.ns + d];
+#pragma unroll
+            for (int m = 0; m < L; ++m)
+                if (s + m == n - 1) x[m] = v;
+        }
+    }
+}
+
+// Load / store the cnt (<= 0: none) nodes of a thread's chunk at line +
+// m*stride; nodes past cnt load as 0.  Full chunks take the unpredicated path.
+template <typename T, int L>
+__device__ __forceinline__ void load_line_chunk(T (&x)[L], const T* line, size_t stride, int cnt) {
+    if (cnt >= L) {
+#pragma unroll
+        for (int m = 0; m < L; ++m) x[m] = line[(size_t)m * stride];
+    } else {
+#pragma unroll
+        for (int m = 0; m < L; ++m) x[m] = m < cnt ? line[(size_t)m * stride] : T(0);
+    }
+}
+
+template <typename T, int L>
+__device__ __forceinline__ void store_line_chunk(const T (&x)[L], T* line, size_t stride, int cnt) {
+    if (cnt >= L) {
+#pragma unroll
+        for (int m = 0; m < L; ++m) line[(size_t)m * stride] = x[m];
+    } else {
+#pragma unroll
+        for (int m = 0; m < L; ++m)
+            if (m < cnt) line[(size_t)m * stride] = x[m];
+    }
+}
+
+// blockDim.x = TX*P with P = ceil(n/L); grid = (ceil(ncols/TX), ns).
+// Dynamic LDS: (2*P*L + TX*P + P) * sizeof(T).
 template <typename T, int L, int AXIS, bool DEV, int TX>
 __global__ __launch_bounds__(1024) void k_sweep_col(SweepArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -600,54 +658,39 @@ __global__ __launch_bounds__(1024) void k_sweep_col(SweepArgs<T> a) {
     const int t = threadIdx.x;
     const int lane = t % TX;
     const int p = t / TX;
-    const int lb = xcd_block(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
-    const int d = a.d0 + lb / gridDim.x;
+    // uniform over the block: keep the tile and substrate in scalar registers
+    const int lb = __builtin_amdgcn_readfirstlane(
+        xcd_block(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y));
+    const int d = __builtin_amdgcn_readfirstlane(a.d0 + lb / gridDim.x);
     stage_coefs<T>(s_inv, s_cp, a.inv + (long long)d * n, a.cp + (long long)d * n, n, P * L, t,
                    blockDim.x);
-    const long long c = a.first + (long long)(lb % gridDim.x) * TX + lane;
-    const bool act = c < a.first + a.count;
-    long long base = 0, stride;
+    const int c = (int)a.first + __builtin_amdgcn_readfirstlane(lb % gridDim.x) * TX + lane;
+    const bool act = c < (int)(a.first + a.count);
     int ci = 0, co = 0;  // i and the other transverse coordinate (k for y, j for z)
     if (act) {
-        co = (int)(c / nx);
-        ci = (int)(c - (long long)co * nx);
+        co = c / nx;
+        ci = c - co * nx;
     }
-    if (AXIS == 1) {
-        base = (long long)d * a.N + (long long)co * ny * nx + ci;
-        stride = nx;
-    } else {
-        base = (long long)d * a.N + c;
-        stride = (long long)nx * ny;
-    }
+    T* ub = a.u + (long long)d * a.N;
+    const unsigned base = AXIS == 1 ? (unsigned)(co * ny * nx + ci) : (unsigned)c;
+    const size_t stride = AXIS == 1 ? (size_t)nx : (size_t)nx * ny;
     const int s = p * L;
+    // this thread's chunk: node m at line + m*stride (a uniform stride, so
+    // each address is one 64-bit add)
+    T* line = ub + (size_t)base + (size_t)s * stride;
+    const int cnt = act ? n - s : 0;
     T x[L];
-#pragma unroll
-    for (int m = 0; m < L; ++m) x[m] = (act && s + m < n) ? a.u[base + (s + m) * stride] : T(0);
+    load_line_chunk<T, L>(x, line, stride, cnt);
     if (DEV) {
+        // padded nodes (past the line end) do not influence the line
         const T bt = a.beta[d];
 #pragma unroll
-        for (int m = 0; m < L; ++m) x[m] = (s + m < n) ? x[m] - bt : T(0);
+        for (int m = 0; m < L; ++m) x[m] = x[m] - bt;
     }
     __syncthreads();  // coefficient staging complete
     chunk_solve<T, L>(x, act, p, P, lane, TX, s_inv, s_cp, a.k[d], s_z, s_g);
-    const bool dact = (a.active >> d) & 1ull;
-    if (act && dact) {
-        const bool lineb = AXIS == 1 ? (ci == 0 || ci == nx - 1 || co == 0 || co == nz - 1)
-                                     : (ci == 0 || ci == nx - 1 || co == 0 || co == ny - 1);
-        const int s_lo = AXIS == 1 ? dir_slot(ci, 0, co, nx, ny, nz, a.reg)
-                                   : dir_slot(ci, co, 0, nx, ny, nz, a.reg);
-        const int s_hi = AXIS == 1 ? dir_slot(ci, n - 1, co, nx, ny, nz, a.reg)
-                                   : dir_slot(ci, co, n - 1, nx, ny, nz, a.reg);
-        const int s_in = AXIS == 1 ? dir_slot(ci, -1, co, nx, ny, nz, a.reg)
-                                   : dir_slot(ci, co, -1, nx, ny, nz, a.reg);
-        dirichlet_row<T, L>(x, s, n, lineb, a.dval[s_lo * a.ns + d], a.dval[s_hi * a.ns + d],
-                            a.dval[s_in * a.ns + d]);
-    }
-    if (act) {
-#pragma unroll
-        for (int m = 0; m < L; ++m)
-            if (s + m < n) a.u[base + (s + m) * stride] = x[m];
-    }
+    if (act && ((a.active >> d) & 1ull)) col_dirichlet<T, L, AXIS>(x, a, d, ci, co, s, n, p, P);
+    store_line_chunk<T, L>(x, line, stride, cnt);
 }
 
 }  // namespace bfa
