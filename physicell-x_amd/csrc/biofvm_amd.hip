@@ -31,6 +31,10 @@ constexpr int kLs[] = {4, 8, 16, 32, 64};
 
 }  // namespace
 
+// Elements of zeroed slack after every device field: the x sweeps load whole
+// row chunks (up to 31 nodes past a row's end) and never store there.
+constexpr size_t kFieldSlack = 64;
+
 struct bfa_env {
     int nx = 0, ny = 0, nz = 0, ns = 0;
     double dx = 0.0;
@@ -325,7 +329,6 @@ int launch_xw_L(bfa_env* e, hipStream_t st, const Win& w) {
     const int P = (nx + L - 1) / L;
     const int RW = 64 / P;
     const long long groups = (w.count + RW - 1) / RW;
-    const bool vec = ((size_t)nx * sizeof(T)) % 16 == 0 && ((size_t)L * sizeof(T)) % 16 == 0;
     // one resident wave of blocks fills the device; each wavefront then
     // strides over its row groups with the next one's loads in flight
     const long long want = std::max<long long>(1, (long long)e->cus * e->xw_occ / w.nd);
@@ -335,10 +338,7 @@ int launch_xw_L(bfa_env* e, hipStream_t st, const Win& w) {
     a.first = w.first;
     a.count = w.count;
     dim3 grid(bx, (unsigned)w.nd);
-    if (vec)
-        hipLaunchKernelGGL((bfa::k_sweep_x_wave<T, L, DEV, true>), grid, dim3(256), 0, st, a);
-    else
-        hipLaunchKernelGGL((bfa::k_sweep_x_wave<T, L, DEV, false>), grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL((bfa::k_sweep_x_wave<T, L, DEV>), grid, dim3(256), 0, st, a);
     LAUNCH_CHECK(e, "x sweep (wave)");
     return BFA_OK;
 }
@@ -392,12 +392,9 @@ int raise_lds_limits(bfa_env* e) {
             HIP_TRY(e, hipFuncSetAttribute(fn[a], hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)need[a]));
     if (e->R == 0) {
-        const bool vec = ((size_t)e->nx * sizeof(T)) % 16 == 0 && ((size_t)e->L[0] * sizeof(T)) % 16 == 0;
         const void* fw = nullptr;
-#define BFA_XW(Lv)                                                                   \
-    if (e->L[0] == Lv)                                                               \
-        fw = vec ? (const void*)bfa::k_sweep_x_wave<T, Lv, DEV, true>                 \
-                 : (const void*)bfa::k_sweep_x_wave<T, Lv, DEV, false>;
+#define BFA_XW(Lv) \
+    if (e->L[0] == Lv) fw = (const void*)bfa::k_sweep_x_wave<T, Lv, DEV>;
         BFA_XW(4) BFA_XW(8) BFA_XW(16) BFA_XW(32)
 #undef BFA_XW
         e->xw_occ = fw ? occupancy_blocks(fw, 256) : 1;
@@ -704,9 +701,10 @@ int bfa_create(bfa_env** out, int nx, int ny, int nz, int ns, double dx, int pre
         fail(e, BFA_ERR_HIP, "event creation failed: %s", hipGetErrorString(st));
         return bail(BFA_ERR_HIP);
     }
-    int rc = dalloc(e, &e->u, (size_t)e->N * ns * e->tsize);
+    // the field plus a zeroed slack that row-chunk loads may read past the end
+    int rc = dalloc(e, &e->u, ((size_t)e->N * ns + kFieldSlack) * e->tsize);
     if (rc) return bail(rc);
-    st = hipMemsetAsync(e->u, 0, (size_t)e->N * ns * e->tsize, e->stream);
+    st = hipMemsetAsync(e->u, 0, ((size_t)e->N * ns + kFieldSlack) * e->tsize, e->stream);
     if (st != hipSuccess) {
         fail(e, BFA_ERR_HIP, "hipMemset failed: %s", hipGetErrorString(st));
         return bail(BFA_ERR_HIP);

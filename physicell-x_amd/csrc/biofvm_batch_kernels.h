@@ -40,11 +40,19 @@ struct EnvDesc {
 struct XItem {
     int env, d, row0, nrows;
 };
-// y / z sweep item: columns [c0, c0 + ntiles*TX) of substrate d of member
-// env, solved as ntiles tiles of TX columns.
+// y / z sweep item: columns [c0, c0 + ntiles*W) of substrate d of member
+// env, solved as ntiles tiles of W = col_tile_width(block, P) columns.
 struct ColItem {
     int env, d, c0, ntiles;
 };
+
+// Columns per tile for lines of P chunks in a block of T threads: the
+// widest multiple of 16 with width * P <= T (T >= 16 * P is required).
+// Short lines get wide tiles, so every item fills the block.
+__host__ __device__ inline int col_tile_width(int T, int P) {
+    const int w = (T / P) & ~15;
+    return w < 16 ? 16 : w;
+}
 
 template <typename T>
 __device__ __forceinline__ SweepArgs<T> args_of(const EnvDesc<T>& e, int axis) {
@@ -99,22 +107,18 @@ __global__ __launch_bounds__(256) void kb_sweep_x(const EnvDesc<T>* __restrict__
         const int valid = lane_on ? (nx - s < L ? nx - s : L) : 0;
         XCoef<T, L> c;
         x_coef_setup<T, L, DEV>(a, w.d, p, P, s, valid, c);
-        const bool vec = ((nx * (int)sizeof(T)) & 15) == 0 && ((L * (int)sizeof(T)) & 15) == 0;
-        const int rend = w.row0 + w.nrows;
-        if (vec)
-            x_rows_pipelined<T, L, DEV, true>(a, w.d, w.row0, rend, RW, rw, lane_on, p, P, s, valid, c);
-        else
-            x_rows_pipelined<T, L, DEV, false>(a, w.d, w.row0, rend, RW, rw, lane_on, p, P, s, valid, c);
+        x_rows_pipelined<T, L, DEV>(a, w.d, w.row0, w.row0 + w.nrows, RW, rw, lane_on, p, P, s, valid, c);
     }
 }
 
-// Persistent batched y (AXIS 1) or z (AXIS 2) sweep.  Each block takes
-// ColItems from ticket[0]; blockDim.x = TX * PMAX with PMAX >= ceil(n/L) for
-// every member.  The ticket of the next item is drawn while the current one
-// runs (two LDS slots, alternating), and the column tiles of an item are
-// pipelined: the loads of tile q+1 are issued before tile q is solved.
-// Dynamic LDS: (2*PMAX*L + TX*PMAX + PMAX) * sizeof(T) + 16.
-template <typename T, int L, int AXIS, bool DEV, int TX>
+// Persistent batched y (AXIS 1) or z (AXIS 2) sweep over every member.  Each
+// block of T threads takes ColItems from ticket[0]; an item whose lines have
+// P chunks runs as tiles of W = col_tile_width(T, P) columns x P chunk rows
+// (T >= 16 * PMAX).  The ticket of the next item is drawn while the current
+// one runs (two LDS slots, alternating), the coefficients are staged once per
+// item, and the loads of tile q+1 are issued before tile q is solved.
+// Dynamic LDS: (2*PMAX*L + T + PMAX) * sizeof(T) + 16.
+template <typename T, int L, int AXIS, bool DEV>
 __global__ __launch_bounds__(1024) void kb_sweep_col(const EnvDesc<T>* __restrict__ envs,
                                                      const ColItem* __restrict__ items,
                                                      const int* __restrict__ nitems,
@@ -124,11 +128,9 @@ __global__ __launch_bounds__(1024) void kb_sweep_col(const EnvDesc<T>* __restric
     T* s_inv = reinterpret_cast<T*>(smem);
     T* s_cp = s_inv + PMAX * L;
     T* s_z = s_cp + PMAX * L;
-    T* s_g = s_z + (size_t)TX * PMAX;
+    T* s_g = s_z + blockDim.x;
     int* s_item = reinterpret_cast<int*>(s_g + PMAX);
     const int t = threadIdx.x;
-    const int lane = t % TX;
-    const int p = t / TX;
     const int n_items = *nitems;
     if (t == 0) s_item[0] = atomicAdd(ticket, 1);
     for (int slot = 0;; slot ^= 1) {
@@ -146,6 +148,9 @@ __global__ __launch_bounds__(1024) void kb_sweep_col(const EnvDesc<T>* __restric
         const int nx = a.nx, ny = a.ny, nz = a.nz;
         const int n = AXIS == 1 ? ny : nz;
         const int P = (n + L - 1) / L;
+        const int W = col_tile_width(blockDim.x, P);
+        const int lane = t % W;
+        const int p = t / W;
         const int d = w.d;
         stage_coefs<T>(s_inv, s_cp, a.inv + (long long)d * n, a.cp + (long long)d * n, n, P * L, t,
                        blockDim.x);
@@ -179,7 +184,7 @@ __global__ __launch_bounds__(1024) void kb_sweep_col(const EnvDesc<T>* __restric
             const int ci = ci_n, co = co_n;
             const bool act = act_n;
             if (q + 1 < w.ntiles) {
-                act_n = col_of(w.c0 + (q + 1) * TX + lane, ci_n, co_n, base_n);
+                act_n = col_of(w.c0 + (q + 1) * W + lane, ci_n, co_n, base_n);
                 load_line_chunk<T, L>(xn, ub + (size_t)base_n + (size_t)s * stride, stride,
                                       act_n ? n - s : 0);
             }
@@ -189,7 +194,7 @@ __global__ __launch_bounds__(1024) void kb_sweep_col(const EnvDesc<T>* __restric
                 for (int m = 0; m < L; ++m) x[m] = x[m] - bt;
             }
             __syncthreads();  // staging done / the previous tile's fold readers are done
-            chunk_solve<T, L>(x, act, p, P, lane, TX, s_inv, s_cp, kd, s_z, s_g);
+            chunk_solve<T, L>(x, act, p, P, lane, W, s_inv, s_cp, kd, s_z, s_g);
             if (act && dact) col_dirichlet<T, L, AXIS>(x, a, d, ci, co, s, n, p, P);
             store_line_chunk<T, L>(x, ub + (size_t)base + (size_t)s * stride, stride, act ? n - s : 0);
         }

@@ -256,49 +256,52 @@ __global__ __launch_bounds__(1024) void k_sweep_x(SweepArgs<T> a, int R, int log
 // A row of nx nodes is cut into P = ceil(nx/L) chunks owned by P consecutive
 // lanes of one wavefront, so a wave holds RW = 64/P whole rows and never
 // synchronises with other waves.  Each lane reads its L contiguous nodes with
-// 16-byte vector loads when the row pitch allows (VEC), the chunk carries are
+// 16-byte vector loads (at any element alignment), the chunk carries are
 // combined with a Hillis-Steele scan of affine maps y -> a*y + b over the
 // lanes of the row (shuffles, log2(P) steps each way), and rows are
 // prefetched one group ahead so that loads of group g+1 overlap the solve of
 // group g.  The lane's 1/den and c' values stay in registers for the whole
 // kernel.  Grid: (blocks, ns) with 256-thread blocks; each wave walks row
 // groups g = wave_id, wave_id + total_waves, ...
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef double f64x2 __attribute__((ext_vector_type(2)));
+// 16-byte vectors with only the element's alignment: a row chunk starts at
+// an arbitrary element, and gfx950 takes unaligned global vector accesses.
+typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef double f64x2u __attribute__((ext_vector_type(2), aligned(8)));
 template <typename T>
 struct Vec16;
 template <>
 struct Vec16<float> {
-    using type = f32x4;
+    using type = f32x4u;
     static constexpr int n = 4;
 };
 template <>
 struct Vec16<double> {
-    using type = f64x2;
+    using type = f64x2u;
     static constexpr int n = 2;
 };
 
-template <typename T, int L, bool VEC>
-__device__ __forceinline__ void load_chunk(T (&x)[L], const T* __restrict__ src, int valid) {
-    if (VEC && valid == L) {
-        using V = typename Vec16<T>::type;
-        constexpr int W = Vec16<T>::n;
-        const V* v = reinterpret_cast<const V*>(src);
+// Load the L nodes of a row chunk with 16-byte vector loads.  Nodes past the
+// row end are read too (the next row's, or the zeroed slack that follows
+// every field allocation) and are ignored by the solve.
+template <typename T, int L>
+__device__ __forceinline__ void load_chunk(T (&x)[L], const T* __restrict__ src) {
+    using V = typename Vec16<T>::type;
+    constexpr int W = Vec16<T>::n;
+    static_assert(L % W == 0, "chunk length must fill 16-byte vectors");
+    const V* v = reinterpret_cast<const V*>(src);
 #pragma unroll
-        for (int q = 0; q < L / W; ++q) {
-            const V t = __builtin_nontemporal_load(v + q);
+    for (int q = 0; q < L / W; ++q) {
+        const V t = __builtin_nontemporal_load(v + q);
 #pragma unroll
-            for (int w = 0; w < W; ++w) x[q * W + w] = t[w];
-        }
-    } else {
-#pragma unroll
-        for (int m = 0; m < L; ++m) x[m] = m < valid ? src[m] : T(0);
+        for (int w = 0; w < W; ++w) x[q * W + w] = t[w];
     }
 }
 
-template <typename T, int L, bool VEC>
+// Store the first `valid` nodes of a row chunk: whole chunks with vector
+// stores, the row's last chunk node by node.
+template <typename T, int L>
 __device__ __forceinline__ void store_chunk(const T (&x)[L], T* __restrict__ dst, int valid) {
-    if (VEC && valid == L) {
+    if (valid == L) {
         using V = typename Vec16<T>::type;
         constexpr int W = Vec16<T>::n;
         V* v = reinterpret_cast<V*>(dst);
@@ -425,13 +428,13 @@ struct XRow {
     bool rowb;           // the row is on a y or z face and Dirichlet is on
 };
 
-template <typename T, int L, bool VEC>
+template <typename T, int L>
 __device__ __forceinline__ void x_row_fetch(const SweepArgs<T>& a, int d, int r, bool have, int s,
                                             int valid, const XCoef<T, L>& c, XRow<T, L>& q) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const T* ub = a.u + (long long)d * a.N;
     if (have)
-        load_chunk<T, L, VEC>(q.v, ub + (long long)r * nx + s, valid);
+        load_chunk<T, L>(q.v, ub + (long long)r * nx + s);
     else {
 #pragma unroll
         for (int m = 0; m < L; ++m) q.v[m] = T(0);
@@ -468,7 +471,7 @@ __device__ __forceinline__ void x_row_dirichlet(T (&cur)[L], int nx, int s, bool
     }
 }
 
-template <typename T, int L, bool DEV, bool VEC>
+template <typename T, int L, bool DEV>
 __device__ __forceinline__ void x_row_solve(const SweepArgs<T>& a, int d, int r, bool have, int p,
                                             int P, int s, int valid, const XCoef<T, L>& c,
                                             XRow<T, L>& q) {
@@ -519,30 +522,30 @@ __device__ __forceinline__ void x_row_solve(const SweepArgs<T>& a, int d, int r,
     }
     if (have) {
         x_row_dirichlet<T, L>(cur, nx, s, have, c, q);
-        store_chunk<T, L, VEC>(cur, a.u + (long long)d * a.N + (long long)r * nx + s, valid);
+        store_chunk<T, L>(cur, a.u + (long long)d * a.N + (long long)r * nx + s, valid);
     }
 }
 
 // Rows [r0, r1) of substrate d in row groups of RW, pipelined: the next
 // group's loads are issued before the current group is solved.
-template <typename T, int L, bool DEV, bool VEC>
+template <typename T, int L, bool DEV>
 __device__ __forceinline__ void x_rows_pipelined(const SweepArgs<T>& a, int d, int r0, int r1,
                                                  int RW, int rw, bool lane_on, int p, int P, int s,
                                                  int valid, const XCoef<T, L>& c) {
     XRow<T, L> nxt;
-    x_row_fetch<T, L, VEC>(a, d, r0 + rw, lane_on && r0 + rw < r1, s, valid, c, nxt);
+    x_row_fetch<T, L>(a, d, r0 + rw, lane_on && r0 + rw < r1, s, valid, c, nxt);
     for (int g = r0; g < r1; g += RW) {
         XRow<T, L> cur = nxt;
         const int r = g + rw;
         if (g + RW < r1) {
             const int rn = r + RW;
-            x_row_fetch<T, L, VEC>(a, d, rn, lane_on && rn < r1, s, valid, c, nxt);
+            x_row_fetch<T, L>(a, d, rn, lane_on && rn < r1, s, valid, c, nxt);
         }
-        x_row_solve<T, L, DEV, VEC>(a, d, r, lane_on && r < r1, p, P, s, valid, c, cur);
+        x_row_solve<T, L, DEV>(a, d, r, lane_on && r < r1, p, P, s, valid, c, cur);
     }
 }
 
-template <typename T, int L, bool DEV, bool VEC>
+template <typename T, int L, bool DEV>
 __global__ __launch_bounds__(256) void k_sweep_x_wave(SweepArgs<T> a) {
     const int nx = a.nx;
     const int P = (nx + L - 1) / L;
@@ -570,7 +573,7 @@ __global__ __launch_bounds__(256) void k_sweep_x_wave(SweepArgs<T> a) {
     XRow<T, L> nxt;
     {
         const int r = row0 + g * RW + rw;
-        x_row_fetch<T, L, VEC>(a, d, r, g < ngroups && lane_on && r < rows, s, valid, c, nxt);
+        x_row_fetch<T, L>(a, d, r, g < ngroups && lane_on && r < rows, s, valid, c, nxt);
     }
     for (; g < ngroups; g += wstride) {
         XRow<T, L> cur = nxt;
@@ -578,9 +581,9 @@ __global__ __launch_bounds__(256) void k_sweep_x_wave(SweepArgs<T> a) {
         const int gn = g + wstride;
         if (gn < ngroups) {
             const int rn = row0 + gn * RW + rw;
-            x_row_fetch<T, L, VEC>(a, d, rn, lane_on && rn < rows, s, valid, c, nxt);
+            x_row_fetch<T, L>(a, d, rn, lane_on && rn < rows, s, valid, c, nxt);
         }
-        x_row_solve<T, L, DEV, VEC>(a, d, r, lane_on && r < rows, p, P, s, valid, c, cur);
+        x_row_solve<T, L, DEV>(a, d, r, lane_on && r < rows, p, P, s, valid, c, cur);
     }
 }
 
