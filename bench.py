@@ -173,8 +173,12 @@ def ensemble_members(n_env, seed):
     return out
 
 
-def run_ensemble(pkg, n_env=256, steps=100, repeats=3, seed=4, sequential=True):
-    """Batched ensemble vs sequential plain solves of the same members."""
+def run_ensemble(pkg, n_env=256, steps=100, repeats=3, seed=4, sequential=True, tune=True):
+    """Batched ensemble vs sequential plain solves of the same members.  The
+    batch kernel shapes are autotuned on the batch itself, after which every
+    member is reset to its initial state."""
+    import importlib
+    AT = importlib.import_module("physicellx_amd.autotune")
     mem = ensemble_members(n_env, seed)
     updates = sum(m["nx"] * m["ny"] * m["nz"] * m["ns"] for m in mem) * steps
     res = {"members": n_env, "steps": steps, "updates_per_run": updates}
@@ -188,6 +192,11 @@ def run_ensemble(pkg, n_env=256, steps=100, repeats=3, seed=4, sequential=True):
             b.fill(mid, m["dval"])
             b.set_agents(mid, m["voxel"], m["c1"], m["c2"], m["e2"])
             ids.append(mid)
+        tuned = None
+        if tune:
+            tuned = AT.autotune_batch(b, 0.01)
+            for mid, m in zip(ids, mem):
+                b.fill(mid, m["dval"])
         b.step(0.01, steps)  # the configured run from the initial state (also the warm-up)
         b.synchronize()
         fields[prec] = [b.download(ids[k], "soa") for k in (0, n_env // 2, n_env - 1)]
@@ -199,7 +208,8 @@ def run_ensemble(pkg, n_env=256, steps=100, repeats=3, seed=4, sequential=True):
             lat.append(time.perf_counter() - t0)
         lat = np.array(lat)
         res[prec] = {"value": updates / float(np.median(lat)), "p50_s": float(np.percentile(lat, 50)),
-                     "p99_s": float(np.percentile(lat, 99)), "runs": repeats, "tiles": list(b.tiles())}
+                     "p99_s": float(np.percentile(lat, 99)), "runs": repeats, "tiles": list(b.tiles()),
+                     "autotune_trials": tuned["trials"] if tuned else 0}
         b.close()
     errs = [float(np.max(np.abs(a - c)) / np.max(np.abs(c))) for a, c in zip(fields["fp32"], fields["fp64"])]
     res["fp32_vs_fp64_max_rel_err"] = max(errs)

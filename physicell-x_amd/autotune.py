@@ -134,3 +134,46 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=3, cache_path=
 def apply(env, res):
     env.set_tiles(res["Lx"], res["Ly"], res["Lz"], res["rows_x"], res["cols_y"], res["cols_z"])
     env.set_slab(res.get("slab", 0))
+
+
+# ---- batches ------------------------------------------------------------------
+BATCH_X_L = (8, 16)
+BATCH_COL = [(L, TX) for L in (8, 16, 32) for TX in (16, 32, 64, 128)]
+
+
+def autotune_batch(batch, dt=0.01, steps=4, verbose=False):
+    """Pick the batch kernel shapes for the batch's current membership.
+
+    Times ``steps`` whole batched steps (one captured graph replay) for every
+    admissible (Lx, Ly = Lz, columns per tile) and keeps the fastest; shapes the
+    batch rejects (lines too long for the block) are skipped.  Every trial
+    advances all members by ``steps`` steps: tune before loading the state to
+    be solved, or reload it afterwards.  Returns {"tiles": (Lx, Ly, Lz, cols_y,
+    cols_z), "ms_per_step", "trials"}.
+    """
+    import time
+
+    trials = []
+    best = None
+    for lx in BATCH_X_L:
+        for L, tx in BATCH_COL:
+            shape = (lx, L, L, tx, tx)
+            try:
+                batch.set_tiles(*shape)
+            except Exception:
+                continue
+            batch.step(dt, steps)  # capture and first replay of the shape
+            batch.synchronize()
+            t0 = time.perf_counter()
+            batch.step(dt, steps)
+            batch.synchronize()
+            ms = (time.perf_counter() - t0) / steps * 1e3
+            trials.append({"tiles": list(shape), "ms_per_step": ms})
+            if verbose:
+                print(json.dumps(trials[-1]), flush=True)
+            if best is None or ms < best[1]:
+                best = (shape, ms)
+    if best is None:
+        raise RuntimeError("no admissible batch kernel shape")
+    batch.set_tiles(*best[0])
+    return {"tiles": best[0], "ms_per_step": best[1], "trials": len(trials)}

@@ -117,9 +117,15 @@ __global__ __launch_bounds__(256) void kb_sweep_x(const EnvDesc<T>* __restrict__
 // (T >= 16 * PMAX).  The ticket of the next item is drawn while the current
 // one runs (two LDS slots, alternating), the coefficients are staged once per
 // item, and the loads of tile q+1 are issued before tile q is solved.
-// Dynamic LDS: (2*PMAX*L + T + PMAX) * sizeof(T) + 16.
+// Dynamic LDS: (2*PMAX*L + T + PMAX) * sizeof(T) + 16.  Chunks of 128 bytes
+// or more take blocks of at most kb_col_max_threads(L * sizeof(T)) = 512
+// threads, which leaves them the registers to stay out of scratch.
+__host__ __device__ constexpr int kb_col_max_threads(int chunk_bytes) {
+    return chunk_bytes >= 128 ? 512 : 1024;
+}
+
 template <typename T, int L, int AXIS, bool DEV>
-__global__ __launch_bounds__(1024) void kb_sweep_col(const EnvDesc<T>* __restrict__ envs,
+__global__ __launch_bounds__(kb_col_max_threads(L * sizeof(T))) void kb_sweep_col(const EnvDesc<T>* __restrict__ envs,
                                                      const ColItem* __restrict__ items,
                                                      const int* __restrict__ nitems,
                                                      int* __restrict__ ticket, int PMAX,
