@@ -89,11 +89,15 @@ class Server:
         self.max_edge = int(max_edge)
         self.dt = float(dt)
         self.clock = clock
-        # chunk lengths that cover max_edge: x rows in <= 64 lanes, y/z lines
-        # in <= 32 chunk rows of 32 columns
+        # shapes that cover max_edge: x rows in <= 64 lanes; y/z lines in
+        # chunks of 16 nodes, as many columns per tile (<= 64) as the block
+        # holds (1024 threads, 512 for chunks of 128 bytes or more)
         lx = 8 if max_edge <= 512 else 16
-        lyz = 8 if max_edge <= 256 else 16
-        self.batch.set_tiles(lx, lyz, lyz, 32, 32)
+        lyz = 16 if max_edge <= 512 else 32
+        limit = 512 if lyz * (4 if precision == "fp32" else 8) >= 128 else 1024
+        p = -(-max_edge // lyz)
+        cols = next((c for c in (64, 32, 16) if c * p <= limit), 16)
+        self.batch.set_tiles(lx, lyz, lyz, cols, cols)
         self.queue = collections.deque()
         self.active = {}
         self.done = []
