@@ -194,11 +194,20 @@ int bfa_batch_set_member_steps(bfa_batch *batch, int member_id, int steps);
 int bfa_batch_step(bfa_batch *batch, double dt, int nsteps);
 int bfa_batch_synchronize(bfa_batch *batch);
 int bfa_batch_total_mass(bfa_batch *batch, int member_id, double *out);
-/* Kernel shapes: x chunk length (8, 16, 32), y / z chunk length (4, 8, 16)
- * and columns per block (16, 32, 64); 0 = automatic.  get writes
- * {Lx, Ly, Lz, cols_y, cols_z}. */
+/* Kernel shapes: x chunk length (8, 16, 32), y / z chunk length (4, 8, 16,
+ * 32) and columns per tile of the longest lines (16, 32, 64, 128); 0 =
+ * automatic.  get writes {Lx, Ly, Lz, cols_y, cols_z}. */
 int bfa_batch_set_tiles(bfa_batch *batch, int Lx, int Ly, int Lz, int cols_y, int cols_z);
 int bfa_batch_get_tiles(const bfa_batch *batch, int *out5);
+/* Fused x + y sweeps (default on): members whose k-plane fits a 128 KiB
+ * on-chip buffer are swept along x and y plane by plane in LDS, reading and
+ * writing the field once for both sweeps.  Same per-line arithmetic as the
+ * separate sweeps except the x chunking (5 nodes).  No reference
+ * counterpart: the reference sweeps each axis over the whole grid
+ * (BioFVM/BioFVM_solvers.cpp:352-730). */
+int bfa_batch_set_fusion(bfa_batch *batch, int enabled);
+/* Number of live members that take the fused sweep. */
+int bfa_batch_fused_members(const bfa_batch *batch);
 
 #ifdef __cplusplus
 }

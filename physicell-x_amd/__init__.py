@@ -105,6 +105,8 @@ def lib():
     L.bfa_batch_total_mass.argtypes = [_vp, ctypes.c_int, _dp]
     L.bfa_batch_set_tiles.argtypes = [_vp] + [ctypes.c_int] * 5
     L.bfa_batch_get_tiles.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
+    L.bfa_batch_set_fusion.argtypes = [_vp, ctypes.c_int]
+    L.bfa_batch_fused_members.argtypes = [_vp]
     _lib = L
     return L
 
@@ -411,3 +413,11 @@ class Batch:
         v = (ctypes.c_int * 5)()
         self._check(lib().bfa_batch_get_tiles(self._h, v), "batch_get_tiles")
         return tuple(v)
+
+    def set_fusion(self, enabled=True):
+        """Fused x + y sweeps of k-planes in LDS for the members whose plane
+        fits (default on); off: every member takes the separate sweeps."""
+        self._check(lib().bfa_batch_set_fusion(self._h, int(bool(enabled))), "batch_set_fusion")
+
+    def fused_members(self):
+        return int(lib().bfa_batch_fused_members(self._h))

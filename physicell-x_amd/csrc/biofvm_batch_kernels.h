@@ -207,6 +207,181 @@ __global__ __launch_bounds__(kb_col_max_threads(L * sizeof(T))) void kb_sweep_co
     }
 }
 
+// ---- fused x + y sweeps of whole k-planes held in LDS ------------------------
+// A k-plane of a member (nx*ny values, contiguous in the device layout) is
+// read from HBM once into LDS, swept along x (Dirichlet D0, the row solves,
+// D1) and along y (the column solves, D2) there, and written back once, in
+// place of the two HBM round trips of separate x and y sweeps.  Used for the
+// members whose plane fits the block's plane buffer; the others take the
+// separate sweeps.  Per-line arithmetic is that of the separate sweeps
+// (x_row_compute, chunk_solve, col_dirichlet) with x chunks of LX nodes and
+// y chunks of LY nodes.
+
+// xy item: planes [k0, k0 + nk) of substrate d of member env.
+struct XYItem {
+    int env, d, k0, nk;
+};
+
+// Threads per block and 16-byte vectors per thread of the plane buffer
+// (the next plane's prefetch registers): planes of up to threads * vecs * 16
+// bytes (128 KiB) are fused.
+template <typename T>
+struct XYShape {
+    static constexpr int threads = sizeof(T) == 4 ? 512 : 256;
+    static constexpr int vecs = sizeof(T) == 4 ? 16 : 32;
+    static constexpr int max_bytes = threads * vecs * 16;
+};
+
+// LX is odd, so the lanes of a row (LX nodes apart) read distinct LDS banks.
+// Dynamic LDS: (PLANE + LX) + 2*PYMAX*LY + threads + PYMAX values of T, + 16.
+template <typename T, int LX, int LY, bool DEV>
+__global__ __launch_bounds__(XYShape<T>::threads) void kb_sweep_xy(
+    const EnvDesc<T>* __restrict__ envs, const XYItem* __restrict__ items,
+    const int* __restrict__ nitems, int* __restrict__ ticket, int PLANE, int PYMAX, int step) {
+    using V = typename Vec16<T>::type;  // unaligned 16-byte vector (global side)
+    typedef T VA __attribute__((ext_vector_type(16 / sizeof(T))));  // aligned (LDS side)
+    constexpr int VW = 16 / sizeof(T);
+    constexpr int NV = XYShape<T>::vecs;
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* plane = reinterpret_cast<T*>(smem);
+    T* s_inv = plane + ((PLANE + LX + VW - 1) / VW) * VW;
+    T* s_cp = s_inv + PYMAX * LY;
+    T* s_z = s_cp + PYMAX * LY;
+    T* s_g = s_z + blockDim.x;
+    int* s_item = reinterpret_cast<int*>(s_g + PYMAX);
+    const int t = threadIdx.x;
+    const int nthr = blockDim.x;
+    const int wave = t >> 6, lane64 = t & 63, nwaves = nthr >> 6;
+    const int n_items = *nitems;
+    if (t == 0) s_item[0] = atomicAdd(ticket, 1);
+    for (int slot = 0;; slot ^= 1) {
+        __syncthreads();  // s_item[slot] written; the previous item's LDS readers are done
+        const int it = __builtin_amdgcn_readfirstlane(s_item[slot]);
+        if (it >= n_items) break;
+        int nxt = 0;
+        if (t == 0) nxt = atomicAdd(ticket, 1);
+        const XYItem w = items[it];
+        if (step >= envs[w.env].steps) {  // uniform over the block
+            if (t == 0) s_item[slot ^ 1] = nxt;
+            continue;
+        }
+        const SweepArgs<T> ax = args_of<T>(envs[w.env], 0);
+        const SweepArgs<T> ay = args_of<T>(envs[w.env], 1);
+        const int nx = ax.nx, ny = ax.ny, d = w.d;
+        const int nxy = nx * ny;
+        const int nvec = (nxy + VW - 1) / VW;
+        // x: lane p of a row owns nodes [p*LX, p*LX + LX); RW rows per wave
+        const int PX = (nx + LX - 1) / LX;
+        const int RW = 64 / PX;
+        const int rw = lane64 / PX;
+        const int px = lane64 - rw * PX;
+        const bool xlane = rw < RW;
+        const int sx = px * LX;
+        const int xvalid = xlane ? (nx - sx < LX ? nx - sx : LX) : 0;
+        // y: tiles of W columns x PY chunk rows
+        const int PY = (ny + LY - 1) / LY;
+        const int W = col_tile_width(nthr, PY);
+        const int ly = t % W, py = t / W;
+        const int sy = py * LY;
+        stage_coefs<T>(s_inv, s_cp, ay.inv + (long long)d * ny, ay.cp + (long long)d * ny, ny, PY * LY,
+                       t, nthr);
+        const T bt = DEV ? ay.beta[d] : T(0);
+        const T ky = ay.k[d];
+        const bool dact = (ay.active >> d) & 1ull;
+        if (t == 0) s_item[slot ^ 1] = nxt;
+        // the plane's 16-byte vectors v = t + q*nthr, q < NV; the last one may
+        // run past the plane end (into the next plane or the field slack),
+        // which lands in the plane buffer's padding
+        V pre[NV];
+        const V* src = reinterpret_cast<const V*>(ax.u + (long long)d * ax.N + (long long)w.k0 * nxy);
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            const int v = t + q * nthr;
+            if (v < nvec) pre[q] = __builtin_nontemporal_load(src + v);
+        }
+        for (int k = w.k0; k < w.k0 + w.nk; ++k) {
+            __syncthreads();  // the previous plane's store phase has read the buffer
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                const int v = t + q * nthr;
+                if (v < nvec) {
+                    VA o;
+#pragma unroll
+                    for (int e = 0; e < VW; ++e) o[e] = pre[q][e];
+                    reinterpret_cast<VA*>(plane)[v] = o;
+                }
+            }
+            if (k + 1 < w.k0 + w.nk) {  // in flight while this plane is solved
+                const V* nsrc = reinterpret_cast<const V*>(ax.u + (long long)d * ax.N + (long long)(k + 1) * nxy);
+#pragma unroll
+                for (int q = 0; q < NV; ++q) {
+                    const int v = t + q * nthr;
+                    if (v < nvec) pre[q] = __builtin_nontemporal_load(nsrc + v);
+                }
+            }
+            __syncthreads();
+            // x solves of the plane's rows (row r of the sweep numbering is k*ny + j);
+            // the lane's coefficient state is rebuilt per plane so that it is
+            // not held through the y solves
+            XCoef<T, LX> cx;
+            x_coef_setup<T, LX, DEV>(ax, d, px, PX, sx, xvalid, cx);
+            for (int j0 = wave * RW; j0 < ny; j0 += nwaves * RW) {
+                const int j = j0 + rw;
+                const bool have = xlane && j < ny;
+                XRow<T, LX> q;
+#pragma unroll
+                for (int m = 0; m < LX; ++m) q.v[m] = have ? plane[j * nx + sx + m] : T(0);
+                x_row_dir_fetch<T, LX>(ax, d, k * ny + j, have, cx, q);
+                x_row_compute<T, LX, DEV>(nx, have, px, PX, sx, cx, q);
+                if (have) {
+#pragma unroll
+                    for (int m = 0; m < LX; ++m)
+                        if (m < xvalid) plane[j * nx + sx + m] = q.v[m];
+                }
+            }
+            __syncthreads();  // the plane's rows are solved
+            // y solves of the plane's columns, W at a time
+            for (int c0 = 0; c0 < nx; c0 += W) {
+                const int i = c0 + ly;
+                const bool act = py < PY && i < nx;
+                T x[LY];
+                const T* col = plane + sy * nx + i;
+#pragma unroll
+                for (int m = 0; m < LY; ++m) x[m] = (act && sy + m < ny) ? col[m * nx] - bt : T(0);
+                __syncthreads();  // the previous tile's fold readers are done
+                chunk_solve<T, LY>(x, act, py, PY, ly, W, s_inv, s_cp, ky, s_z, s_g);
+                if (act && dact) col_dirichlet<T, LY, 1>(x, ay, d, i, k, sy, ny, py, PY);
+                if (act) {
+                    T* cw = plane + sy * nx + i;
+#pragma unroll
+                    for (int m = 0; m < LY; ++m)
+                        if (sy + m < ny) cw[m * nx] = x[m];
+                }
+            }
+            __syncthreads();
+            // store the plane: whole vectors, the plane's last partial one by element
+            T* dst = ax.u + (long long)d * ax.N + (long long)k * nxy;
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                const int v = t + q * nthr;
+                if (v < nvec) {
+                    const VA o = reinterpret_cast<const VA*>(plane)[v];
+                    if ((v + 1) * VW <= nxy) {
+                        V u;
+#pragma unroll
+                        for (int e = 0; e < VW; ++e) u[e] = o[e];
+                        __builtin_nontemporal_store(u, reinterpret_cast<V*>(dst) + v);
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < VW; ++e)
+                            if (v * VW + e < nxy) dst[v * VW + e] = o[e];
+                    }
+                }
+            }
+        }
+    }
+}
+
 // Agent entries over the whole batch: entry q updates the value at absolute
 // device address u_of[q] with its agents' maps a0..a1 (ascending agent
 // order): rho = (rho + c1)/c2 + e2 in FP64; FP32 members lift v to

@@ -428,17 +428,12 @@ struct XRow {
     bool rowb;           // the row is on a y or z face and Dirichlet is on
 };
 
+// The per-row Dirichlet part of x_row_fetch: rowb and the face values of
+// row r (= k*ny + j) when it lies on a y or z face.
 template <typename T, int L>
-__device__ __forceinline__ void x_row_fetch(const SweepArgs<T>& a, int d, int r, bool have, int s,
-                                            int valid, const XCoef<T, L>& c, XRow<T, L>& q) {
+__device__ __forceinline__ void x_row_dir_fetch(const SweepArgs<T>& a, int d, int r, bool have,
+                                                const XCoef<T, L>& c, XRow<T, L>& q) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const T* ub = a.u + (long long)d * a.N;
-    if (have)
-        load_chunk<T, L>(q.v, ub + (long long)r * nx + s);
-    else {
-#pragma unroll
-        for (int m = 0; m < L; ++m) q.v[m] = T(0);
-    }
     q.f_lo = q.f_hi = q.f_in = T(0);
     q.rowb = false;
     if (have && c.dact) {
@@ -451,6 +446,19 @@ __device__ __forceinline__ void x_row_fetch(const SweepArgs<T>& a, int d, int r,
             q.f_in = a.dval[dir_slot(-1, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
         }
     }
+}
+
+template <typename T, int L>
+__device__ __forceinline__ void x_row_fetch(const SweepArgs<T>& a, int d, int r, bool have, int s,
+                                            int valid, const XCoef<T, L>& c, XRow<T, L>& q) {
+    const T* ub = a.u + (long long)d * a.N;
+    if (have)
+        load_chunk<T, L>(q.v, ub + (long long)r * a.nx + s);
+    else {
+#pragma unroll
+        for (int m = 0; m < L; ++m) q.v[m] = T(0);
+    }
+    x_row_dir_fetch<T, L>(a, d, r, have, c, q);
 }
 
 // Dirichlet pass of one row chunk (see x_row_fetch).
@@ -471,11 +479,10 @@ __device__ __forceinline__ void x_row_dirichlet(T (&cur)[L], int nx, int s, bool
     }
 }
 
+// Dirichlet D0, the row solve and Dirichlet D1 on the chunk held in q.v.
 template <typename T, int L, bool DEV>
-__device__ __forceinline__ void x_row_solve(const SweepArgs<T>& a, int d, int r, bool have, int p,
-                                            int P, int s, int valid, const XCoef<T, L>& c,
-                                            XRow<T, L>& q) {
-    const int nx = a.nx;
+__device__ __forceinline__ void x_row_compute(int nx, bool have, int p, int P, int s,
+                                              const XCoef<T, L>& c, XRow<T, L>& q) {
     T(&cur)[L] = q.v;
     x_row_dirichlet<T, L>(cur, nx, s, have, c, q);
     // past the row end 1/den = c' = 0, so whatever sits there leaves the
@@ -520,10 +527,15 @@ __device__ __forceinline__ void x_row_solve(const SweepArgs<T>& a, int d, int r,
         xn = fma(-c.cp[m], xn, cur[m]);
         cur[m] = xn;
     }
-    if (have) {
-        x_row_dirichlet<T, L>(cur, nx, s, have, c, q);
-        store_chunk<T, L>(cur, a.u + (long long)d * a.N + (long long)r * nx + s, valid);
-    }
+    if (have) x_row_dirichlet<T, L>(cur, nx, s, have, c, q);
+}
+
+template <typename T, int L, bool DEV>
+__device__ __forceinline__ void x_row_solve(const SweepArgs<T>& a, int d, int r, bool have, int p,
+                                            int P, int s, int valid, const XCoef<T, L>& c,
+                                            XRow<T, L>& q) {
+    x_row_compute<T, L, DEV>(a.nx, have, p, P, s, c, q);
+    if (have) store_chunk<T, L>(q.v, a.u + (long long)d * a.N + (long long)r * a.nx + s, valid);
 }
 
 // Rows [r0, r1) of substrate d in row groups of RW, pipelined: the next

@@ -108,6 +108,32 @@ def test_batch_member_equals_plain_solve(pkg, precision):
     assert per_substrate_err(got, plain, p.ns) <= (1e-13 if precision == "fp64" else 2e-6)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fused_xy_sweep_matches_separate_sweeps_and_oracle(pkg, precision):
+    """Members whose k-plane fits on chip take the fused x+y sweep; the others
+    (here the 200x170 plane, and in FP64 also 160x160) the separate sweeps.
+    Both agree with each other and with the oracle."""
+    ps = problems(seed0=700)[:6] + [make_problem(200, 170, 12, 2, seed=777, n_agents=50,
+                                                 dirichlet="mixed", export=5.0, collide=3)]
+    steps = 3
+    out = {}
+    for fuse in (True, False):
+        with pkg.Batch(precision) as b:
+            b.set_fusion(fuse)
+            ids = [add(b, p) for p in ps]
+            b.step(ps[0].dt, steps)
+            nf = b.fused_members()
+            if fuse:
+                assert 0 < nf < len(ps), nf
+            else:
+                assert nf == 0
+            out[fuse] = [b.download(m) for m in ids]
+    for k, p in enumerate(ps):
+        ref = oracle_run(p, steps)
+        assert per_substrate_err(out[True][k], ref, p.ns) <= TOL[precision], (k, precision)
+        assert per_substrate_err(out[True][k], out[False][k], p.ns) <= (1e-13 if precision == "fp64" else 2e-6)
+
+
 def test_member_steps_and_membership_changes(pkg):
     ps = problems(seed0=400)[:5]
     with pkg.Batch("fp64") as b:
