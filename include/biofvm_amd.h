@@ -199,7 +199,7 @@ int bfa_batch_total_mass(bfa_batch *batch, int member_id, double *out);
  * automatic.  get writes {Lx, Ly, Lz, cols_y, cols_z}. */
 int bfa_batch_set_tiles(bfa_batch *batch, int Lx, int Ly, int Lz, int cols_y, int cols_z);
 int bfa_batch_get_tiles(const bfa_batch *batch, int *out5);
-/* Fused x + y sweeps (default on): members whose k-plane fits a 128 KiB
+/* Fused x + y sweeps (default off): members whose k-plane fits a 128 KiB
  * on-chip buffer are swept along x and y plane by plane in LDS, reading and
  * writing the field once for both sweeps.  Same per-line arithmetic as the
  * separate sweeps except the x chunking (5 nodes).  No reference

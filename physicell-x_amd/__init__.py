@@ -416,7 +416,7 @@ class Batch:
 
     def set_fusion(self, enabled=True):
         """Fused x + y sweeps of k-planes in LDS for the members whose plane
-        fits (default on); off: every member takes the separate sweeps."""
+        fits; off (the default): every member takes the separate sweeps."""
         self._check(lib().bfa_batch_set_fusion(self._h, int(bool(enabled))), "batch_set_fusion")
 
     def fused_members(self):

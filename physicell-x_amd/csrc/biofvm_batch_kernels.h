@@ -329,8 +329,10 @@ __global__ __launch_bounds__(XYShape<T>::threads) void kb_sweep_xy(
                 const int j = j0 + rw;
                 const bool have = xlane && j < ny;
                 XRow<T, LX> q;
+                // nodes past the row end read as 0 (past the plane's last row
+                // the buffer holds stale LDS contents, which may be NaN)
 #pragma unroll
-                for (int m = 0; m < LX; ++m) q.v[m] = have ? plane[j * nx + sx + m] : T(0);
+                for (int m = 0; m < LX; ++m) q.v[m] = (have && m < xvalid) ? plane[j * nx + sx + m] : T(0);
                 x_row_dir_fetch<T, LX>(ax, d, k * ny + j, have, cx, q);
                 x_row_compute<T, LX, DEV>(nx, have, px, PX, sx, cx, q);
                 if (have) {
