@@ -87,6 +87,7 @@ def make_env(pkg, w, n, ns, precision, tiles=None):
         env.set_tiles(tiles["Lx"], tiles["Ly"], tiles["Lz"], tiles["rows_x"], tiles["cols_y"],
                       tiles["cols_z"])
         env.set_slab(tiles.get("slab", 0))
+        env.set_substrate_streams(tiles.get("substrate_streams", 1))
     env.set_coefficients(w["D"], w["decay"])
     env.set_dirichlet(np.ones(ns, np.uint8), np.ones(6, np.uint8), np.tile(w["dval"], (6, 1)))
     env.upload(w["field"], "soa")
@@ -95,19 +96,21 @@ def make_env(pkg, w, n, ns, precision, tiles=None):
 
 
 def time_gpu(env, dt, steps, warmup):
+    """K timed steps as the library runs them (bfa_step: one captured graph
+    per step, replayed), bracketed by device synchronisation; then K more
+    steps launched kernel by kernel with HIP events around every kernel, for
+    the per-kernel times."""
     env.step(dt, warmup)
     env.synchronize()
     t0 = time.perf_counter()
-    ms = env.time_kernels(dt, steps)  # K steps, HIP events around every kernel
-    env.synchronize()
-    wall = time.perf_counter() - t0
-    # graph-replayed steps, for reference
-    env.synchronize()
-    t1 = time.perf_counter()
     env.step(dt, steps)
     env.synchronize()
-    graph_wall = time.perf_counter() - t1
-    return wall, ms, graph_wall
+    wall = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    ms = env.time_kernels(dt, steps)
+    env.synchronize()
+    eager_wall = time.perf_counter() - t1
+    return wall, ms, eager_wall
 
 
 def load_traffic(kernel_key):
@@ -397,7 +400,7 @@ def main():
     ap.add_argument("--mode", default="default", choices=["default", "single", "ensemble", "serve", "large"])
     ap.add_argument("--requests", type=int, default=40)
     ap.add_argument("--no-seq", action="store_true", help="ensemble mode: skip the sequential plain solves")
-    ap.add_argument("--tiles", default="", help="fixed FP32 shapes Lx,Ly,Lz,rows_x,cols_y,cols_z[,slab] "
+    ap.add_argument("--tiles", default="", help="fixed FP32 shapes Lx,Ly,Lz,rows_x,cols_y,cols_z[,slab[,streams]] "
                     "instead of autotuning (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -423,7 +426,7 @@ def main():
     if args.tiles:
         v = [int(x) for x in args.tiles.split(",")]
         tiles["fp32"] = dict(Lx=v[0], Ly=v[1], Lz=v[2], rows_x=v[3], cols_y=v[4], cols_z=v[5],
-                             slab=v[6] if len(v) > 6 else 0)
+                             slab=v[6] if len(v) > 6 else 0, substrate_streams=v[7] if len(v) > 7 else 1)
     elif not args.no_tune:
         import importlib
         at = importlib.import_module("physicellx_amd.autotune")
@@ -449,7 +452,7 @@ def main():
         errs = [float(np.max(np.abs(f32[d] - f64[d])) / np.max(np.abs(f64[d]))) for d in range(ns)]
         drift = [float(abs(m32[d] - m64[d]) / abs(m64[d])) for d in range(ns)]
         res["fp64"] = dict(value=updates * args.steps / wall64, ms_per_step=1e3 * wall64 / args.steps,
-                           graph_ms_per_step=1e3 * gwall64 / args.steps,
+                           eager_ms_per_step=1e3 * gwall64 / args.steps,
                            kernel_ms=dict(x=ms64[0], y=ms64[1], z=ms64[2], agents=ms64[3]))
         res["fp32_vs_fp64"] = dict(max_rel_err_per_substrate=errs, mass_drift_per_substrate=drift,
                                    steps=args.warmup + 2 * args.steps,
@@ -487,9 +490,10 @@ def main():
         "kernel_ms": dict(x=ms32[0], y=ms32[1], z=ms32[2], agents=ms32[3]),
         "step_alg_frac_of_peak": (updates * 6 * 4) / (wall32 / args.steps) / 1e9 / PEAK_HBM_GBS,
         "copy_kernel_GBps": copy_gbs,
-        "graph_ms_per_step": 1e3 * gwall32 / args.steps,
+        "eager_ms_per_step": 1e3 * gwall32 / args.steps,
         "tiles": dict(Lx=tiles32[0], Ly=tiles32[1], Lz=tiles32[2], rows_x=tiles32[3],
                       cols_y=tiles32[4], cols_z=tiles32[5], slab=slab32,
+                      substrate_streams=(tiles["fp32"] or {}).get("substrate_streams", 1),
                       autotuned=not args.no_tune),
     }
     out.update(res)

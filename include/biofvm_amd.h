@@ -153,6 +153,18 @@ int bfa_get_tiles(const bfa_env *env, int *out6);
 int bfa_set_slab(bfa_env *env, int planes);
 int bfa_get_slab(const bfa_env *env);
 
+/* Sweep schedule across substrates: groups == 2 sweeps the first and the
+ * second half of the substrates (x, y, z each) on two parallel streams, so
+ * that the tail of each kernel overlaps the other half's work; groups == 1
+ * (default) sweeps all substrates with one launch per axis.  Per-line
+ * arithmetic is the same, so the result is bit-identical.  Ignored while a
+ * slab schedule is set.  bfa_time_kernels always times the single-group
+ * sweeps kernel by kernel; bfa_step uses the schedule.  No reference
+ * counterpart (the reference sweeps substrates inside one loop nest,
+ * BioFVM/BioFVM_solvers.cpp:352-777). */
+int bfa_set_substrate_streams(bfa_env *env, int groups);
+int bfa_get_substrate_streams(const bfa_env *env);
+
 /* ---- batches of independent microenvironments -------------------------------
  *
  * A batch advances many microenvironments of different shapes and substrate

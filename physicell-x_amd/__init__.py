@@ -105,6 +105,8 @@ def lib():
     L.bfa_batch_total_mass.argtypes = [_vp, ctypes.c_int, _dp]
     L.bfa_batch_set_tiles.argtypes = [_vp] + [ctypes.c_int] * 5
     L.bfa_batch_get_tiles.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
+    L.bfa_set_substrate_streams.argtypes = [_vp, ctypes.c_int]
+    L.bfa_get_substrate_streams.argtypes = [_vp]
     L.bfa_batch_set_fusion.argtypes = [_vp, ctypes.c_int]
     L.bfa_batch_fused_members.argtypes = [_vp]
     _lib = L
@@ -239,6 +241,14 @@ class Microenvironment:
 
     def slab(self):
         return lib().bfa_get_slab(self._h)
+
+    def set_substrate_streams(self, groups):
+        """1: one launch per sweep for all substrates; 2: two substrate halves
+        swept on parallel streams (same results)."""
+        self._check(lib().bfa_set_substrate_streams(self._h, int(groups)), "set_substrate_streams")
+
+    def substrate_streams(self):
+        return lib().bfa_get_substrate_streams(self._h)
 
     # -- field transfer -----------------------------------------------------------
     def upload(self, field, layout="ref"):

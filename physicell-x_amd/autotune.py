@@ -9,8 +9,9 @@ sweep kernels on the GPU (HIP events around each launch, via
   form with R rows per tile and chunk length L;
 * y and z sweeps: TX in {16, 32, 64} x-columns per block and chunk length L,
   with TX * ceil(n/L) <= 1024 threads, chosen per axis;
-* schedule: whole-volume sweeps, or x/y sweeps in k-slabs of a few planes
-  (chosen by whole-step time).
+* schedule: whole-volume sweeps, the two substrate halves swept on parallel
+  streams, or x/y sweeps in k-slabs of a few planes (chosen by graph-replayed
+  whole-step time).
 
 Results are cached per (nx, ny, nz, ns, precision) in memory and, when a path
 is given, in a JSON file.
@@ -42,6 +43,19 @@ def _time(env, dt, reps, shape):
         return None
     env.time_kernels(dt, 1)  # first launch of a shape pays code-object load
     return env.time_kernels(dt, reps)
+
+
+def _step_ms(env, dt, reps):
+    """Mean graph-replayed step time (ms) over 2*reps + 2 steps."""
+    import time
+
+    env.step(dt, 2)  # capture the schedule's graph
+    env.synchronize()
+    n = 2 * reps + 2
+    t0 = time.perf_counter()
+    env.step(dt, n)
+    env.synchronize()
+    return (time.perf_counter() - t0) / n * 1e3
 
 
 def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=3, cache_path=None,
@@ -91,26 +105,28 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=3, cache_path=
                     best_z = (ms[2], L, TX)
         shape = (best_x[1], best_y[1], best_z[1], best_x[2], best_y[2], best_z[2])
         env.set_tiles(*shape)
-        env.time_kernels(dt, 1)
-        step0 = env.time_kernels(dt, reps)[4]
-        best_slab = (step0, 0)
-        trials.append({"axis": "schedule", "slab": 0, "ms_step": step0})
+        # schedules, by graph-replayed step time: whole-volume sweeps, the two
+        # substrate halves on parallel streams, x/y k-slabs
+        cands = [(0, 1)]
+        if ns > 1:
+            cands.append((0, 2))
         if best_x[2] == 0:
             plane = nx * ny * (4 if precision in ("fp32", "f32", 1) else 8)
-            for nbytes in SLAB_BYTES:
-                planes = max(1, nbytes // plane)
-                if planes >= nz:
-                    continue
-                env.set_slab(planes)
-                env.time_kernels(dt, 1)
-                t = env.time_kernels(dt, reps)[4]
-                trials.append({"axis": "schedule", "slab": planes, "ms_step": t})
-                if t < best_slab[0]:
-                    best_slab = (t, planes)
-            env.set_slab(0)
+            cands += [(max(1, nb // plane), 1) for nb in SLAB_BYTES if max(1, nb // plane) < nz]
+        best_sched = None
+        for slab, split in cands:
+            env.set_slab(slab)
+            env.set_substrate_streams(split)
+            t = _step_ms(env, dt, reps)
+            trials.append({"axis": "schedule", "slab": slab, "substrate_streams": split, "ms_step": t})
+            if best_sched is None or t < best_sched[0]:
+                best_sched = (t, slab, split)
+        env.set_slab(0)
+        env.set_substrate_streams(1)
         res = {"Lx": best_x[1], "rows_x": best_x[2], "Ly": best_y[1], "cols_y": best_y[2],
-               "Lz": best_z[1], "cols_z": best_z[2], "slab": best_slab[1],
-               "ms": {"x": best_x[0], "y": best_y[0], "z": best_z[0], "step": best_slab[0]},
+               "Lz": best_z[1], "cols_z": best_z[2], "slab": best_sched[1],
+               "substrate_streams": best_sched[2],
+               "ms": {"x": best_x[0], "y": best_y[0], "z": best_z[0], "step": best_sched[0]},
                "trials": trials}
     finally:
         env.close()
@@ -134,6 +150,7 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=3, cache_path=
 def apply(env, res):
     env.set_tiles(res["Lx"], res["Ly"], res["Lz"], res["rows_x"], res["cols_y"], res["cols_z"])
     env.set_slab(res.get("slab", 0))
+    env.set_substrate_streams(res.get("substrate_streams", 1))
 
 
 # ---- batches ------------------------------------------------------------------

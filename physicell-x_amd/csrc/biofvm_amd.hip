@@ -92,6 +92,7 @@ struct bfa_env {
     int R = 0;
     int TX[3] = {0, 64, 64};
     int slab = 0;  // k-planes per x/y slab pair (0: whole-volume sweeps)
+    int split = 1;  // substrate groups swept on parallel streams (1 or 2)
 
     hipStream_t aux[2] = {nullptr, nullptr};
     hipEvent_t ev_fork = nullptr;
@@ -478,6 +479,26 @@ int enqueue_xy_slabs(bfa_env* e) {
     return BFA_OK;
 }
 
+// The three sweeps of two substrate groups on the two side streams, so that
+// each kernel's tail overlaps the other group's work.  Forks from and joins
+// back to e->stream.
+int enqueue_split(bfa_env* e) {
+    HIP_TRY(e, hipEventRecord(e->ev_fork, e->stream));
+    const int h = (e->ns + 1) / 2;
+    for (int g = 0; g < 2; ++g) {
+        hipStream_t st = e->aux[g];
+        HIP_TRY(e, hipStreamWaitEvent(st, e->ev_fork, 0));
+        const int d0 = g == 0 ? 0 : h, nd = g == 0 ? h : e->ns - h;
+        for (int a = 0; a < 3; ++a) {
+            int rc = launch_sweep_win(e, a, st, Win{d0, nd, 0, axis_lines(e, a)});
+            if (rc) return rc;
+        }
+        HIP_TRY(e, hipEventRecord(e->ev_join[g], st));
+    }
+    for (int g = 0; g < 2; ++g) HIP_TRY(e, hipStreamWaitEvent(e->stream, e->ev_join[g], 0));
+    return BFA_OK;
+}
+
 int launch_agents(bfa_env* e) {
     if (e->nseg == 0) return BFA_OK;
     const long long work = (long long)e->nseg * e->ns;
@@ -560,6 +581,7 @@ int ensure_precompute(bfa_env* e, double dt) {
 }
 
 int enqueue_diffusion(bfa_env* e) {
+    if (e->split > 1 && e->ns > 1 && e->slab == 0) return enqueue_split(e);
     if (e->slab > 0 && e->R == 0) {
         int rc = enqueue_xy_slabs(e);
         if (rc) return rc;
@@ -1157,6 +1179,16 @@ int bfa_set_slab(bfa_env* e, int planes) {
 }
 
 int bfa_get_slab(const bfa_env* e) { return e ? e->slab : -1; }
+
+int bfa_set_substrate_streams(bfa_env* e, int groups) {
+    if (!e) return BFA_ERR_ARG;
+    if (groups != 1 && groups != 2) return fail(e, BFA_ERR_ARG, "substrate stream groups must be 1 or 2, got %d", groups);
+    e->split = groups;
+    invalidate_graph(e);
+    return BFA_OK;
+}
+
+int bfa_get_substrate_streams(const bfa_env* e) { return e ? e->split : -1; }
 
 int bfa_copy_kernel(size_t bytes, int reps, double* ms) {
     if (!ms || reps < 1 || bytes < 16) return BFA_ERR_ARG;

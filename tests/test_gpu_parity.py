@@ -302,6 +302,36 @@ def test_slab_schedule_is_bit_identical(pkg, precision, planes):
         eager.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("ns", [2, 3])
+def test_substrate_stream_schedule_is_bit_identical(pkg, precision, ns):
+    """The two substrate halves swept on parallel streams give exactly the
+    single-launch result, eager and graph; a bad group count is rejected."""
+    p = make_problem(41, 30, 27, ns, seed=43, n_agents=150, dirichlet="mixed", export=5.0)
+    ref = gpu_env(pkg, p, precision)
+    sp = gpu_env(pkg, p, precision)
+    eager = gpu_env(pkg, p, precision)
+    try:
+        sp.set_substrate_streams(2)
+        eager.set_substrate_streams(2)
+        assert sp.substrate_streams() == 2
+        with pytest.raises(pkg.BioFVMError):
+            sp.set_substrate_streams(3)
+        assert sp.substrate_streams() == 2
+        ref.step(p.dt, 3)
+        sp.step(p.dt, 3)
+        for _ in range(3):
+            eager.simulate_diffusion_decay(p.dt)
+            eager.simulate_cell_sources_and_sinks()
+        r = ref.download()
+        assert np.array_equal(sp.download(), r)
+        assert np.array_equal(eager.download(), r)
+    finally:
+        ref.close()
+        sp.close()
+        eager.close()
+
+
 def test_invalid_tiles_are_rejected_and_state_kept(pkg):
     p = make_problem(20, 20, 20, 1, seed=3, dirichlet="off")
     env = gpu_env(pkg, p, "fp64")
