@@ -324,6 +324,36 @@ def run_serving(pkg, n_req=40, seed=5, rate=200.0, budget=200e9, k=10, spot=2):
     return res
 
 
+def run_aux(env, w, updates, reps=10):
+    """The kernels beside the step, on the same FP32 400^3 x 4 field: the
+    whole-field gradient (device kernel time; reads the field, writes 3 FP64
+    values per voxel-substrate), the gradient at the agents' voxels through
+    the C ABI (host round trip included), and one bulk source/sink update
+    (reads the field and two FP64 rate fields, writes the field)."""
+    out = {}
+    gms = env.time_gradient(reps)
+    gbytes = updates * (4 + 3 * 8)
+    out["gradient"] = {"kernel_ms": gms, "alg_bytes": gbytes, "GBps": gbytes / (gms * 1e-3) / 1e9}
+    vox = np.ascontiguousarray(w["voxel"], np.int32)
+    env.gradients_at(vox)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        env.gradients_at(vox)
+    out["gradient_at_agents"] = {"voxels": int(vox.size), "ms_per_call": (time.perf_counter() - t0) / reps * 1e3}
+    env.set_bulk_rates(np.full(updates, 0.1), np.full(updates, 30.0), np.full(updates, 0.05))
+    env.simulate_bulk_sources_and_sinks(0.01)
+    env.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        env.simulate_bulk_sources_and_sinks(0.01)
+    env.synchronize()
+    bms = (time.perf_counter() - t0) / reps * 1e3
+    bbytes = updates * (4 + 16 + 4)
+    out["bulk_sources"] = {"ms_per_call": bms, "alg_bytes": bbytes, "GBps": bbytes / (bms * 1e-3) / 1e9}
+    env.set_bulk_rates(None, None, None)
+    return out
+
+
 def run_large(pkg, n=800, steps=20, seed=3):
     """800^3 x 2 FP32 scaled-deviation form vs the FP64 path: an oxygen-like
     substrate (D=1e5, decay 0.1, Dirichlet 38) and a signal (D=1e3, decay 0.4,
@@ -458,6 +488,7 @@ def main():
                                    steps=args.warmup + 2 * args.steps,
                                    norm="max|f32 - f64| / max|f64| per substrate")
         env64.close()
+    aux = run_aux(env32, w, updates)
     env32.close()
 
     slab32 = tiles["fp32"]["slab"] if tiles["fp32"] else 0
@@ -495,6 +526,7 @@ def main():
                       cols_y=tiles32[4], cols_z=tiles32[5], slab=slab32,
                       substrate_streams=(tiles["fp32"] or {}).get("substrate_streams", 1),
                       autotuned=not args.no_tune),
+        "aux_kernels": aux,
     }
     out.update(res)
     if args.mode == "default":

@@ -121,6 +121,37 @@ int bfa_apply_dirichlet(bfa_env *env);
 
 /* Total substrate per species, sum of rho*dx^3 over voxels, FP64 [ns]. */
 int bfa_total_mass(bfa_env *env, double *out);
+
+/* Bulk sources and sinks, Microenvironment::simulate_bulk_sources_and_sinks
+ * (BioFVM/BioFVM_microenvironment.cpp:1327-1368).  set_bulk_rates stores the
+ * per-value supply rate S, supply target T and uptake rate U (what the
+ * reference's bulk_supply_rate_function / bulk_supply_target_densities_function /
+ * bulk_uptake_rate_function write, :210-212), [voxel][ns] in BFA_LAYOUT_REF or
+ * [ns][N] in BFA_LAYOUT_SOA; all three NULL releases them.  bulk_sources_and_sinks
+ * applies rho <- (rho + dt*T*S) / (1 + dt*(U + S)) to the resident field with
+ * the reference's roundings; BFA_ERR_STATE if no rates are set. */
+int bfa_set_bulk_rates(bfa_env *env, const double *S, const double *T, const double *U, int layout);
+int bfa_bulk_sources_and_sinks(bfa_env *env, double dt);
+
+/* Gradient field of every substrate, as Microenvironment::
+ * compute_all_gradient_vectors (BioFVM/BioFVM_microenvironment.cpp:1454-1615):
+ * central differences (u[+1] - u[-1]) / (2 dx) inside, one-sided differences
+ * at the two ends of each axis, 0 along an axis of one node.  Computed on the
+ * device from the resident field.  layout BFA_LAYOUT_REF: out[(n*ns + q)*3 + c]
+ * with n the reference voxel index (gradient_vectors[n][q][c]);
+ * BFA_LAYOUT_SOA: out[(q*3 + c)*N + ((k*ny + j)*nx + i)].  FP64 fields give
+ * the reference arithmetic exactly. */
+int bfa_gradients(bfa_env *env, double *out, int layout);
+/* Gradient at n voxels (reference voxel indices), out[(a*ns + q)*3 + c]:
+ * the values compute_all_gradient_vectors gives there, as read through
+ * Microenvironment::nearest_gradient_vector (BioFVM_microenvironment.cpp:1443-1452)
+ * by chemotaxis.  Only the listed voxels are computed. */
+int bfa_gradients_at(bfa_env *env, int n, const int32_t *voxel, double *out);
+/* Mean device time (ms, HIP events) of the whole-field gradient kernel over
+ * reps launches into a device scratch buffer (measurement; no reference
+ * counterpart). */
+int bfa_time_gradient(bfa_env *env, int reps, double *ms);
+
 /* Wait for all work queued on the handle's stream. */
 int bfa_synchronize(bfa_env *env);
 

@@ -62,6 +62,8 @@ def lib():
         L.orc_voxel_indices.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, _ip]
+        L.orc_gradients.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 3 + [_dp, _dp]
+        L.orc_bulk_sources.argtypes = [_dp, ctypes.c_size_t, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -230,3 +232,24 @@ def voxel_indices(pos, bbox, dx, dy, dz, nx, ny, nz):
     lib().orc_voxel_indices(pos.shape[0], _d(pos), _d(bbox), dx, dy, dz, nx, ny, nz,
                             out.ctypes.data_as(_ip))
     return out
+
+
+def gradients(u, nx, ny, nz, ns, dx, dy=None, dz=None):
+    """Gradient field of a reference-layout field: array (nvox, ns, 3)
+    (compute_all_gradient_vectors, BioFVM/BioFVM_microenvironment.cpp:1454-1615)."""
+    dy = dx if dy is None else dy
+    dz = dx if dz is None else dz
+    u = _f64(u)
+    assert u.size == nx * ny * nz * ns
+    g = np.zeros(nx * ny * nz * ns * 3)
+    lib().orc_gradients(nx, ny, nz, ns, dx, dy, dz, _d(u), _d(g))
+    return g.reshape(nx * ny * nz, ns, 3)
+
+
+def bulk_sources(u, ns, dt, S, T, U):
+    """In place: simulate_bulk_sources_and_sinks with evaluated rates
+    (BioFVM/BioFVM_microenvironment.cpp:1327-1368); all arrays reference layout."""
+    S, T, U = _f64(S), _f64(T), _f64(U)
+    assert u.dtype == np.float64 and u.flags.c_contiguous and S.size == T.size == U.size == u.size
+    lib().orc_bulk_sources(_d(u), u.size // ns, ns, float(dt), _d(S), _d(T), _d(U))
+    return u

@@ -19,6 +19,9 @@
  *   - agent constants           BioFVM/BioFVM_basic_agent.cpp:224-274
  *   - secretion / uptake        BioFVM/BioFVM_basic_agent.cpp:509-559,
  *     in serial agent order     BioFVM/BioFVM_microenvironment.cpp:1370-1384
+ *   - gradient field            BioFVM/BioFVM_microenvironment.cpp:1454-1615
+ *     (compute_all_gradient_vectors)
+ *   - bulk sources / sinks      BioFVM/BioFVM_microenvironment.cpp:1327-1368
  *   - mesh node count           BioFVM/BioFVM_mesh.cpp:1471-1473
  *   - nearest voxel             BioFVM/BioFVM_mesh.cpp:1825-1843,
  *                               BioFVM/BioFVM_basic_agent.cpp:177-186,
@@ -446,5 +449,65 @@ void orc_voxel_indices(int n, const double *pos, const double *bbox,
         if (j >= (unsigned)ny) j = ny - 1;
         if (k >= (unsigned)nz) k = nz - 1;
         out[a] = (int)((i * (unsigned)ny + j) * (unsigned)nz + k);
+    }
+}
+
+
+/* Gradient field, compute_all_gradient_vectors
+ * (BioFVM/BioFVM_microenvironment.cpp:1454-1615): per voxel and substrate the
+ * central difference (u[+1] - u[-1]) / (2 h) inside and the one-sided
+ * differences (u[1] - u[0]) / h, (u[n-1] - u[n-2]) / h at the two ends, along
+ * x, y and z, each as one subtraction followed by one division.
+ * g[(n*ns + q)*3 + c].  An axis with a single node gets 0: the reference
+ * skips z when nz == 1 (:1560-1562) and reads outside the field for x or y. */
+void orc_gradients(int nx, int ny, int nz, int ns, double dx, double dy, double dz,
+                   const double *u, double *g)
+{
+    const long long jump[3] = {(long long)ny * nz * ns, (long long)nz * ns, (long long)ns};
+    const int len[3] = {nx, ny, nz};
+    const double h[3] = {dx, dy, dz};
+    const double h2[3] = {2.0 * dx, 2.0 * dy, 2.0 * dz};
+    #pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const long long n = ((long long)i * ny + j) * nz + k;
+                const int idx[3] = {i, j, k};
+                for (int q = 0; q < ns; ++q) {
+                    const long long a = n * ns + q;
+                    for (int c = 0; c < 3; ++c) {
+                        double v;
+                        if (len[c] == 1)
+                            v = 0.0;
+                        else if (idx[c] == 0)
+                            v = (u[a + jump[c]] - u[a]) / h[c];
+                        else if (idx[c] == len[c] - 1)
+                            v = (u[a] - u[a - jump[c]]) / h[c];
+                        else
+                            v = (u[a + jump[c]] - u[a - jump[c]]) / h2[c];
+                        g[a * 3 + c] = v;
+                    }
+                }
+            }
+}
+
+/* Bulk sources and sinks, simulate_bulk_sources_and_sinks
+ * (BioFVM/BioFVM_microenvironment.cpp:1327-1368), with the per-voxel rates
+ * already evaluated: S = supply rate, T = supply target, U = uptake rate, all
+ * [nvox][ns].  Per value: t2 = T*S; rho += dt*t2 (one statement, contracted
+ * by the reference build: fma); t3 = U + S; t3 *= dt; t3 += 1; rho /= t3
+ * (separate statements, separate roundings). */
+void orc_bulk_sources(double *u, size_t nvox, int ns, double dt, const double *S,
+                      const double *T, const double *U)
+{
+    const long long total = (long long)nvox * ns;
+    #pragma omp parallel for schedule(static)
+    for (long long a = 0; a < total; ++a) {
+        const double t2 = T[a] * S[a];
+        double rho = fma(dt, t2, u[a]);
+        double t3 = U[a] + S[a];
+        t3 = t3 * dt;
+        t3 = t3 + 1.0;
+        u[a] = rho / t3;
     }
 }

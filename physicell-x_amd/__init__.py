@@ -105,6 +105,11 @@ def lib():
     L.bfa_batch_total_mass.argtypes = [_vp, ctypes.c_int, _dp]
     L.bfa_batch_set_tiles.argtypes = [_vp] + [ctypes.c_int] * 5
     L.bfa_batch_get_tiles.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
+    L.bfa_set_bulk_rates.argtypes = [_vp, _dp, _dp, _dp, ctypes.c_int]
+    L.bfa_bulk_sources_and_sinks.argtypes = [_vp, ctypes.c_double]
+    L.bfa_time_gradient.argtypes = [_vp, ctypes.c_int, _dp]
+    L.bfa_gradients.argtypes = [_vp, _dp, ctypes.c_int]
+    L.bfa_gradients_at.argtypes = [_vp, ctypes.c_int, _i32, _dp]
     L.bfa_set_substrate_streams.argtypes = [_vp, ctypes.c_int]
     L.bfa_get_substrate_streams.argtypes = [_vp]
     L.bfa_batch_set_fusion.argtypes = [_vp, ctypes.c_int]
@@ -241,6 +246,42 @@ class Microenvironment:
 
     def slab(self):
         return lib().bfa_get_slab(self._h)
+
+    def set_bulk_rates(self, S, T, U, layout="ref"):
+        """Per-value supply rate S, supply target T, uptake rate U for
+        simulate_bulk_sources_and_sinks; None for all three releases them."""
+        if S is None and T is None and U is None:
+            self._check(lib().bfa_set_bulk_rates(self._h, None, None, None, _LAYOUT[layout]), "set_bulk_rates")
+            return
+        n = self.number_of_voxels * self.ns
+        S, T, U = (_f64(a, n) for a in (S, T, U))
+        self._check(lib().bfa_set_bulk_rates(self._h, _d(S), _d(T), _d(U), _LAYOUT[layout]), "set_bulk_rates")
+
+    def simulate_bulk_sources_and_sinks(self, dt):
+        self._check(lib().bfa_bulk_sources_and_sinks(self._h, float(dt)), "bulk_sources_and_sinks")
+
+    def gradients(self, layout="ref"):
+        """Gradient field (compute_all_gradient_vectors): array (nvox, ns, 3) in
+        reference voxel order, or (ns, 3, nvox) device order for layout="soa"."""
+        out = np.empty(self.number_of_voxels * self.ns * 3)
+        self._check(lib().bfa_gradients(self._h, _d(out), _LAYOUT[layout]), "gradients")
+        if layout == "ref":
+            return out.reshape(self.number_of_voxels, self.ns, 3)
+        return out.reshape(self.ns, 3, self.number_of_voxels)
+
+    def time_gradient(self, reps=10):
+        """Mean device time (ms) of the whole-field gradient kernel."""
+        ms = np.zeros(1)
+        self._check(lib().bfa_time_gradient(self._h, int(reps), _d(ms)), "time_gradient")
+        return float(ms[0])
+
+    def gradients_at(self, voxels):
+        """Gradient at the given reference voxel indices: array (n, ns, 3)."""
+        v = np.ascontiguousarray(np.asarray(voxels, dtype=np.int32).ravel())
+        out = np.empty(v.size * self.ns * 3)
+        self._check(lib().bfa_gradients_at(self._h, int(v.size), v.ctypes.data_as(_i32), _d(out)),
+                    "gradients_at")
+        return out.reshape(v.size, self.ns, 3)
 
     def set_substrate_streams(self, groups):
         """1: one launch per sweep for all substrates; 2: two substrate halves

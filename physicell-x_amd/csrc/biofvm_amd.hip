@@ -45,6 +45,8 @@ struct bfa_env {
     int max_lds = 65536;
     int cus = 256;  // compute units of the device
     int xw_occ = 1;  // resident blocks per CU of the chosen wave x-sweep kernel
+    double* bulk_st = nullptr;   // bulk sources: T*S per value (device order), FP64
+    double* bulk_ups = nullptr;  // bulk sources: U + S per value
     hipStream_t stream = nullptr;
     void* u = nullptr;
 
@@ -787,6 +789,8 @@ void bfa_destroy(bfa_env* e) {
     dfree(e->agA);
     dfree(e->agB);
     dfree(e->agC);
+    dfree(e->bulk_st);
+    dfree(e->bulk_ups);
     for (int i = 0; i < 2; ++i) {
         if (e->aux[i]) (void)hipStreamSynchronize(e->aux[i]);
         if (e->aux[i]) (void)hipStreamDestroy(e->aux[i]);
@@ -1127,6 +1131,201 @@ int bfa_synchronize(bfa_env* e) {
     if (!e) return BFA_ERR_ARG;
     HIP_TRY(e, hipStreamSynchronize(e->stream));
     return BFA_OK;
+}
+
+// Enqueue the whole-field gradient into g ([ns][3][N], device).
+int enqueue_gradient(bfa_env* e, double* g) {
+    dim3 grid((unsigned)std::min<long long>(4096, (e->N + 255) / 256), (unsigned)e->ns);
+    if (e->prec == BFA_FP32)
+        hipLaunchKernelGGL(bfa::k_gradient<float>, grid, dim3(256), 0, e->stream, (const float*)e->u,
+                           e->N, e->nx, e->ny, e->nz, e->ns, e->params, e->dx, e->dx, e->dx, g);
+    else
+        hipLaunchKernelGGL(bfa::k_gradient<double>, grid, dim3(256), 0, e->stream, (const double*)e->u,
+                           e->N, e->nx, e->ny, e->nz, e->ns, e->params, e->dx, e->dx, e->dx, g);
+    LAUNCH_CHECK(e, "gradient");
+    return BFA_OK;
+}
+
+int bfa_set_bulk_rates(bfa_env* e, const double* S, const double* T, const double* U, int layout) {
+    if (!e) return BFA_ERR_ARG;
+    if (!S && !T && !U) {
+        dfree(e->bulk_st);
+        dfree(e->bulk_ups);
+        e->bulk_st = e->bulk_ups = nullptr;
+        return BFA_OK;
+    }
+    if (!S || !T || !U) return fail(e, BFA_ERR_ARG, "S, T and U must all be given (or all NULL)");
+    if (layout != BFA_LAYOUT_REF && layout != BFA_LAYOUT_SOA)
+        return fail(e, BFA_ERR_ARG, "unknown layout %d", layout);
+    const size_t total = (size_t)e->N * e->ns;
+    // t2 = T*S and t3 = U + S, each one FP64 rounding, as the reference
+    // forms them (BioFVM/BioFVM_microenvironment.cpp:1344,1351)
+    std::vector<double> h(2 * total);
+    for (size_t a = 0; a < total; ++a) {
+        if (!std::isfinite(S[a]) || !std::isfinite(T[a]) || !std::isfinite(U[a]))
+            return fail(e, BFA_ERR_ARG, "bulk rate %zu is not finite", a);
+        h[a] = T[a] * S[a];
+        h[total + a] = U[a] + S[a];
+    }
+    int rc;
+    if (!e->bulk_st && ((rc = dalloc(e, &e->bulk_st, total * 8)) || (rc = dalloc(e, &e->bulk_ups, total * 8)))) {
+        dfree(e->bulk_st);
+        e->bulk_st = nullptr;
+        return rc;
+    }
+    hipError_t st = hipSuccess;
+    if (layout == BFA_LAYOUT_SOA) {
+        st = hipMemcpyAsync(e->bulk_st, h.data(), total * 8, hipMemcpyHostToDevice, e->stream);
+        if (st == hipSuccess)
+            st = hipMemcpyAsync(e->bulk_ups, h.data() + total, total * 8, hipMemcpyHostToDevice, e->stream);
+        if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+        if (st != hipSuccess) return fail(e, BFA_ERR_HIP, "bulk rate upload failed: %s", hipGetErrorString(st));
+        return BFA_OK;
+    }
+    double* stage = nullptr;
+    double* zo = nullptr;
+    auto cleanup = [&](int code) {
+        (void)hipStreamSynchronize(e->stream);
+        dfree(stage);
+        dfree(zo);
+        return code;
+    };
+    std::vector<double> z1(2 * (size_t)e->ns, 0.0);
+    for (int d = 0; d < e->ns; ++d) z1[e->ns + d] = 1.0;
+    if ((rc = dalloc(e, &stage, total * 8)) || (rc = dalloc(e, &zo, z1.size() * 8))) return cleanup(rc);
+    st = hipMemcpyAsync(zo, z1.data(), z1.size() * 8, hipMemcpyHostToDevice, e->stream);
+    dim3 grid((unsigned)(((long long)e->nz * e->ns + 31) / 32), (unsigned)((e->nx + 31) / 32), (unsigned)e->ny);
+    for (int f = 0; f < 2 && st == hipSuccess; ++f) {
+        st = hipMemcpyAsync(stage, h.data() + f * total, total * 8, hipMemcpyHostToDevice, e->stream);
+        if (st != hipSuccess) break;
+        hipLaunchKernelGGL(bfa::k_ref_to_soa<double>, grid, dim3(32, 8), 0, e->stream, stage,
+                           f == 0 ? e->bulk_st : e->bulk_ups, e->nx, e->ny, e->nz, e->ns, zo, zo + e->ns);
+        st = hipGetLastError();
+        if (st == hipSuccess) st = hipStreamSynchronize(e->stream);  // stage is reused
+    }
+    if (st != hipSuccess) return cleanup(fail(e, BFA_ERR_HIP, "bulk rate upload failed: %s", hipGetErrorString(st)));
+    return cleanup(BFA_OK);
+}
+
+int bfa_bulk_sources_and_sinks(bfa_env* e, double dt) {
+    if (!e) return BFA_ERR_ARG;
+    if (!e->bulk_st) return fail(e, BFA_ERR_STATE, "bulk rates not set (bfa_set_bulk_rates)");
+    if (!(dt >= 0.0) || !std::isfinite(dt)) return fail(e, BFA_ERR_ARG, "dt must be finite and >= 0");
+    dim3 grid((unsigned)std::min<long long>(4096, (e->N + 255) / 256), (unsigned)e->ns);
+    if (e->prec == BFA_FP32)
+        hipLaunchKernelGGL(bfa::k_bulk<float>, grid, dim3(256), 0, e->stream, (float*)e->u, e->N, e->ns,
+                           e->params, e->bulk_st, e->bulk_ups, dt);
+    else
+        hipLaunchKernelGGL(bfa::k_bulk<double>, grid, dim3(256), 0, e->stream, (double*)e->u, e->N,
+                           e->ns, e->params, e->bulk_st, e->bulk_ups, dt);
+    LAUNCH_CHECK(e, "bulk sources");
+    return BFA_OK;
+}
+
+int bfa_gradients(bfa_env* e, double* out, int layout) {
+    if (!e || !out) return fail(e, BFA_ERR_ARG, "NULL argument");
+    if (layout != BFA_LAYOUT_REF && layout != BFA_LAYOUT_SOA)
+        return fail(e, BFA_ERR_ARG, "unknown layout %d", layout);
+    const size_t total = (size_t)e->N * e->ns * 3;
+    double* g = nullptr;
+    double* stage = nullptr;
+    double* bs = nullptr;
+    auto cleanup = [&](int code) {
+        (void)hipStreamSynchronize(e->stream);
+        dfree(g);
+        dfree(stage);
+        dfree(bs);
+        return code;
+    };
+    int rc = dalloc(e, &g, total * 8);
+    if (rc) return rc;
+    if ((rc = enqueue_gradient(e, g))) return cleanup(rc);
+    const double* src = g;
+    if (layout == BFA_LAYOUT_REF) {
+        // (substrate, component) pairs are the fastest index of the
+        // reference layout: transpose [3ns][N] as a field of 3*ns substrates
+        const int n3 = 3 * e->ns;
+        std::vector<double> h(2 * (size_t)n3, 0.0);
+        for (int q = 0; q < n3; ++q) h[n3 + q] = 1.0;
+        if ((rc = dalloc(e, &stage, total * 8)) || (rc = dalloc(e, &bs, h.size() * 8))) return cleanup(rc);
+        hipError_t st = hipMemcpyAsync(bs, h.data(), h.size() * 8, hipMemcpyHostToDevice, e->stream);
+        if (st == hipSuccess) {
+            dim3 grid((unsigned)(((long long)e->nz * n3 + 31) / 32), (unsigned)((e->nx + 31) / 32),
+                      (unsigned)e->ny);
+            hipLaunchKernelGGL(bfa::k_soa_to_ref<double>, grid, dim3(32, 8), 0, e->stream, g, stage,
+                               e->nx, e->ny, e->nz, n3, bs, bs + n3);
+            st = hipGetLastError();
+        }
+        if (st != hipSuccess)
+            return cleanup(fail(e, BFA_ERR_HIP, "gradient layout failed: %s", hipGetErrorString(st)));
+        src = stage;
+    }
+    hipError_t st = hipMemcpyAsync(out, src, total * 8, hipMemcpyDeviceToHost, e->stream);
+    if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+    if (st != hipSuccess) return cleanup(fail(e, BFA_ERR_HIP, "gradient download failed: %s", hipGetErrorString(st)));
+    return cleanup(BFA_OK);
+}
+
+int bfa_time_gradient(bfa_env* e, int reps, double* ms) {
+    if (!e || !ms || reps < 1) return fail(e, BFA_ERR_ARG, "bad argument");
+    double* g = nullptr;
+    int rc = dalloc(e, &g, (size_t)e->N * e->ns * 3 * 8);
+    if (rc) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t st = hipEventCreate(&e0);
+    if (st == hipSuccess) st = hipEventCreate(&e1);
+    if (st == hipSuccess) rc = enqueue_gradient(e, g);  // warm-up
+    if (st == hipSuccess && rc == BFA_OK) st = hipEventRecord(e0, e->stream);
+    for (int r = 0; r < reps && st == hipSuccess && rc == BFA_OK; ++r) rc = enqueue_gradient(e, g);
+    if (st == hipSuccess && rc == BFA_OK) st = hipEventRecord(e1, e->stream);
+    if (st == hipSuccess && rc == BFA_OK) st = hipEventSynchronize(e1);
+    float t = 0.0f;
+    if (st == hipSuccess && rc == BFA_OK) st = hipEventElapsedTime(&t, e0, e1);
+    (void)hipStreamSynchronize(e->stream);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dfree(g);
+    if (rc) return rc;
+    if (st != hipSuccess) return fail(e, BFA_ERR_HIP, "gradient timing failed: %s", hipGetErrorString(st));
+    *ms = t / reps;
+    return BFA_OK;
+}
+
+int bfa_gradients_at(bfa_env* e, int n, const int32_t* voxel, double* out) {
+    if (!e || (n > 0 && (!voxel || !out))) return fail(e, BFA_ERR_ARG, "NULL argument");
+    if (n < 0) return fail(e, BFA_ERR_ARG, "negative voxel count");
+    if (n == 0) return BFA_OK;
+    for (int a = 0; a < n; ++a)
+        if (voxel[a] < 0 || voxel[a] >= e->N)
+            return fail(e, BFA_ERR_ARG, "voxel %d of entry %d outside [0, %lld)", voxel[a], a, e->N);
+    int* dv = nullptr;
+    double* dout = nullptr;
+    auto cleanup = [&](int code) {
+        (void)hipStreamSynchronize(e->stream);
+        dfree(dv);
+        dfree(dout);
+        return code;
+    };
+    int rc;
+    if ((rc = dalloc(e, &dv, (size_t)n * 4)) || (rc = dalloc(e, &dout, (size_t)n * e->ns * 3 * 8)))
+        return cleanup(rc);
+    hipError_t st = hipMemcpyAsync(dv, voxel, (size_t)n * 4, hipMemcpyHostToDevice, e->stream);
+    if (st == hipSuccess) {
+        const unsigned blocks = (unsigned)(((long long)n * e->ns + 255) / 256);
+        if (e->prec == BFA_FP32)
+            hipLaunchKernelGGL(bfa::k_gradient_at<float>, dim3(blocks), dim3(256), 0, e->stream,
+                               (const float*)e->u, e->N, e->nx, e->ny, e->nz, e->ns, e->params, e->dx,
+                               e->dx, e->dx, n, dv, dout);
+        else
+            hipLaunchKernelGGL(bfa::k_gradient_at<double>, dim3(blocks), dim3(256), 0, e->stream,
+                               (const double*)e->u, e->N, e->nx, e->ny, e->nz, e->ns, e->params, e->dx,
+                               e->dx, e->dx, n, dv, dout);
+        st = hipGetLastError();
+    }
+    if (st == hipSuccess) st = hipMemcpyAsync(out, dout, (size_t)n * e->ns * 3 * 8, hipMemcpyDeviceToHost, e->stream);
+    if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+    if (st != hipSuccess) return cleanup(fail(e, BFA_ERR_HIP, "gradient download failed: %s", hipGetErrorString(st)));
+    return cleanup(BFA_OK);
 }
 
 int bfa_time_kernels(bfa_env* e, double dt, int reps, double* ms) {

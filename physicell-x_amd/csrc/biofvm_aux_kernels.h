@@ -302,6 +302,111 @@ __global__ void k_fill_const(T* __restrict__ out, long long N, const double* __r
         out[(long long)d * N + i] = v;
 }
 
+// ---- gradient field --------------------------------------------------------
+// The derivative of substrate d along one axis at device index v (stride
+// between neighbours `st`, position `pos` of n nodes, spacing h): the central
+// difference inside, the one-sided ones at the ends, 0 on an axis of one node;
+// one subtraction then one division, as compute_all_gradient_vectors
+// (BioFVM/BioFVM_microenvironment.cpp:1454-1615).  FP32 fields hold
+// u = b + s*v, so du = s*dv.
+template <typename T>
+__device__ __forceinline__ double grad_axis(const T* __restrict__ ud, long long v, long long st,
+                                            int pos, int n, double h, double s) {
+    if (n == 1) return 0.0;
+    long long hi = v + st, lo = v - st;
+    double hh = 2.0 * h;
+    if (pos == 0) {
+        lo = v;
+        hh = h;
+    } else if (pos == n - 1) {
+        hi = v;
+        hh = h;
+    }
+    if (sizeof(T) == 8) return ((double)ud[hi] - (double)ud[lo]) / hh;
+    return (s * ((double)ud[hi] - (double)ud[lo])) / hh;
+}
+
+// Whole-field gradient: g[(d*3 + c)*N + v] for device index v.
+// Grid: (blocks, ns), grid-stride over v.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gradient(const T* __restrict__ u, long long N, int nx,
+                                                  int ny, int nz, int ns,
+                                                  const double* __restrict__ params, double dx,
+                                                  double dy, double dz, double* __restrict__ g) {
+    const int d = blockIdx.y;
+    const T* ud = u + (long long)d * N;
+    const double s = sizeof(T) == 8 ? 1.0 : params[3 * ns + d];
+    const long long nxy = (long long)nx * ny;
+    double* gd = g + (long long)d * 3 * N;
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < N;
+         v += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(v / nxy);
+        const long long r = v - (long long)k * nxy;
+        const int j = (int)(r / nx);
+        const int i = (int)(r - (long long)j * nx);
+        gd[v] = grad_axis<T>(ud, v, 1, i, nx, dx, s);
+        gd[N + v] = grad_axis<T>(ud, v, nx, j, ny, dy, s);
+        gd[2 * N + v] = grad_axis<T>(ud, v, nxy, k, nz, dz, s);
+    }
+}
+
+// Gradient at listed voxels (reference voxel index n = (i*ny + j)*nz + k):
+// out[(a*ns + d)*3 + c].  One lane per (voxel, substrate).
+template <typename T>
+__global__ __launch_bounds__(256) void k_gradient_at(const T* __restrict__ u, long long N, int nx,
+                                                     int ny, int nz, int ns,
+                                                     const double* __restrict__ params, double dx,
+                                                     double dy, double dz, int n,
+                                                     const int* __restrict__ voxel,
+                                                     double* __restrict__ out) {
+    const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= (long long)n * ns) return;
+    const int a = (int)(w / ns);
+    const int d = (int)(w - (long long)a * ns);
+    const int vr = voxel[a];
+    const int k = vr % nz;
+    const int j = (vr / nz) % ny;
+    const int i = vr / (nz * ny);
+    const long long nxy = (long long)nx * ny;
+    const long long v = (long long)k * nxy + (long long)j * nx + i;
+    const T* ud = u + (long long)d * N;
+    const double s = sizeof(T) == 8 ? 1.0 : params[3 * ns + d];
+    double* o = out + w * 3;
+    o[0] = grad_axis<T>(ud, v, 1, i, nx, dx, s);
+    o[1] = grad_axis<T>(ud, v, nx, j, ny, dy, s);
+    o[2] = grad_axis<T>(ud, v, nxy, k, nz, dz, s);
+}
+
+// ---- bulk sources and sinks ---------------------------------------------------
+// rho <- (rho + dt*ST) / ((U+S)*dt + 1) per value, with ST = T*S and
+// UpS = U + S precomputed in FP64 on the host, in the operation order of
+// simulate_bulk_sources_and_sinks (BioFVM/BioFVM_microenvironment.cpp:1327-1368):
+// one fused multiply-add for rho += dt*t2, separate roundings for t3*dt and
+// t3 + 1 (no contraction).  FP32 fields lift v to u = b + s*v in FP64 and
+// store (u - b)/s.  Grid: (blocks, ns).
+template <typename T>
+__global__ __launch_bounds__(256) void k_bulk(T* __restrict__ u, long long N, int ns,
+                                              const double* __restrict__ params,
+                                              const double* __restrict__ st,
+                                              const double* __restrict__ ups, double dt) {
+    const int d = blockIdx.y;
+    const bool dev = sizeof(T) == 4;
+    const double b = dev ? params[2 * ns + d] : 0.0;
+    const double s = dev ? params[3 * ns + d] : 1.0;
+    const double rs = dev ? params[4 * ns + d] : 1.0;
+    T* ud = u + (long long)d * N;
+    const double* std_ = st + (long long)d * N;
+    const double* upd = ups + (long long)d * N;
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < N;
+         v += (long long)gridDim.x * blockDim.x) {
+        const double x = dev ? fma((double)ud[v], s, b) : (double)ud[v];
+        const double rho = fma(dt, std_[v], x);
+        const double den = __dadd_rn(__dmul_rn(upd[v], dt), 1.0);
+        const double y = rho / den;
+        ud[v] = dev ? (T)((y - b) * rs) : (T)y;
+    }
+}
+
 // Plain 16-byte copy, used only to calibrate HBM byte counters.
 __global__ void k_copy16(const uint4* __restrict__ in, uint4* __restrict__ out, long long n) {
     for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n;

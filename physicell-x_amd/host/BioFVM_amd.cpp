@@ -154,6 +154,7 @@ void Microenvironment::resize_space_uniform(double x_start, double x_end, double
     host_newer = true;
     device_newer = false;
     state_dirty = true;
+    gradients_valid = false;
 }
 
 int Microenvironment::ensure_device() {
@@ -210,12 +211,14 @@ void Microenvironment::download_if_needed() {
 double* Microenvironment::density_vector(int n) {
     download_if_needed();
     host_newer = true;
+    gradients_valid = false;
     return host_field.data() + (size_t)n * number_of_densities();
 }
 
 std::vector<double>& Microenvironment::density_field() {
     download_if_needed();
     host_newer = true;
+    gradients_valid = false;
     return host_field;
 }
 
@@ -242,6 +245,7 @@ void Microenvironment::apply_dirichlet_boundaries_conditions() {
         return;
     }
     device_newer = true;
+    gradients_valid = false;
 }
 
 void Microenvironment::simulate_diffusion_decay(double dt) {
@@ -300,6 +304,59 @@ void Microenvironment::simulate_cell_sources_and_sinks(std::vector<Basic_Agent*>
         return;
     }
     device_newer = true;
+    gradients_valid = false;
+}
+
+void zero_function(Microenvironment* m, int, double* write_destination) {
+    for (int d = 0; d < m->number_of_densities(); ++d) write_destination[d] = 0.0;
+}
+
+void one_function(Microenvironment* m, int, double* write_destination) {
+    for (int d = 0; d < m->number_of_densities(); ++d) write_destination[d] = 1.0;
+}
+
+void Microenvironment::simulate_bulk_sources_and_sinks(double dt) {
+    const int ns = number_of_densities();
+    const size_t total = (size_t)number_of_voxels() * ns;
+    std::vector<double> S(total), T(total), U(total);
+    for (int n = 0; n < number_of_voxels(); ++n) {
+        bulk_supply_rate_function(this, n, &S[(size_t)n * ns]);
+        bulk_supply_target_densities_function(this, n, &T[(size_t)n * ns]);
+        bulk_uptake_rate_function(this, n, &U[(size_t)n * ns]);
+    }
+    if (push_state() || bfa_set_bulk_rates(env, S.data(), T.data(), U.data(), BFA_LAYOUT_REF) ||
+        bfa_bulk_sources_and_sinks(env, dt)) {
+        std::cout << "Error: simulate_bulk_sources_and_sinks: " << (env ? bfa_last_error(env) : last_error.c_str())
+                  << std::endl;
+        return;
+    }
+    device_newer = true;
+    gradients_valid = false;
+}
+
+void Microenvironment::compute_all_gradient_vectors() {
+    // BioFVM/BioFVM_microenvironment.cpp:1454-1615, on the device
+    if (push_state()) {
+        std::cout << "Error: compute_all_gradient_vectors: " << last_error << std::endl;
+        return;
+    }
+    gradient_field.resize((size_t)number_of_voxels() * number_of_densities() * 3);
+    if (bfa_gradients(env, gradient_field.data(), BFA_LAYOUT_REF)) {
+        std::cout << "Error: compute_all_gradient_vectors: " << bfa_last_error(env) << std::endl;
+        return;
+    }
+    gradients_valid = true;
+}
+
+double* Microenvironment::gradient_vector(int n) {
+    if (!gradients_valid) compute_all_gradient_vectors();
+    if (!gradients_valid) return nullptr;
+    return gradient_field.data() + (size_t)n * number_of_densities() * 3;
+}
+
+double* Microenvironment::nearest_gradient_vector(const std::vector<double>& position) {
+    const int n = mesh.nearest_voxel_index(position);
+    return n < 0 ? nullptr : gradient_vector(n);
 }
 
 double Microenvironment::total_mass(int substrate) {
@@ -326,6 +383,7 @@ void diffusion_decay_solver__constant_coefficients_LOD_3D_GPU(Microenvironment& 
         return;
     }
     M.device_newer = true;
+    M.gradients_valid = false;
 }
 
 }  // namespace BioFVM_amd

@@ -32,6 +32,11 @@ namespace BioFVM_amd {
 
 class Microenvironment;
 
+// Rate callbacks writing 0 / 1 for every density of a voxel
+// (BioFVM/BioFVM_microenvironment.cpp:99-110).
+void zero_function(Microenvironment* m, int voxel_index, double* write_destination);
+void one_function(Microenvironment* m, int voxel_index, double* write_destination);
+
 struct Cartesian_Mesh {
     // {x_min, y_min, z_min, x_max, y_max, z_max}, as bounding_box in the reference
     double bounding_box[6] = {0, 0, 0, 1, 1, 1};
@@ -106,6 +111,22 @@ class Microenvironment {
     void set_substrate_dirichlet_activation(int substrate_index, bool new_value);
     void apply_dirichlet_boundaries_conditions();
 
+    // bulk sources and sinks (BioFVM/BioFVM_microenvironment.cpp:1327-1368):
+    // the three rate callbacks (zero_function by default, :167-169) are
+    // evaluated for every voxel on each call, as in the reference, and the
+    // implicit update runs on the device
+    void (*bulk_supply_rate_function)(Microenvironment*, int, double*) = zero_function;
+    void (*bulk_supply_target_densities_function)(Microenvironment*, int, double*) = zero_function;
+    void (*bulk_uptake_rate_function)(Microenvironment*, int, double*) = zero_function;
+    void simulate_bulk_sources_and_sinks(double dt);
+
+    // gradients (BioFVM/BioFVM_microenvironment.cpp:1430-1615): computed on the
+    // device from the resident field; gradient_vector(n) points at the [ns][3]
+    // block of voxel n and recomputes the field's gradients when it changed
+    void compute_all_gradient_vectors();
+    double* gradient_vector(int n);
+    double* nearest_gradient_vector(const std::vector<double>& position);
+
     // stepping
     void simulate_diffusion_decay(double dt);
     void simulate_cell_sources_and_sinks(std::vector<Basic_Agent*>& agents, double dt);
@@ -123,6 +144,8 @@ class Microenvironment {
     int ensure_device();
     int push_state();
     std::vector<double> host_field;
+    std::vector<double> gradient_field;  // [voxel][ns][3]
+    bool gradients_valid = false;
     bfa_env* env = nullptr;
     bool host_newer = true;    // host copy was written since the last upload
     bool device_newer = false; // device copy was stepped since the last download

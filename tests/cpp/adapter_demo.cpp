@@ -6,7 +6,9 @@
 // agents, and runs the main-loop pair of the reference (main.cpp:244 then
 // main.cpp:259): simulate_diffusion_decay(dt) then the agents' secretion and
 // uptake.  Writes the final field (reference layout) and the total mass of
-// every substrate after every step.
+// every substrate after every step, then the gradient field of the final
+// field (compute_all_gradient_vectors, [voxel][ns][3]), then the field after
+// one call of simulate_bulk_sources_and_sinks with the rate callbacks below.
 //
 // Case file (little endian): int32 {nx, ny, nz, ns, steps, n_agents,
 // precision}; float64 {dx, dt, D[ns], decay[ns], active[ns], registered[6],
@@ -20,6 +22,17 @@
 #include "BioFVM_amd.h"
 
 using namespace BioFVM_amd;
+
+// bulk rate callbacks: a function of the voxel index and the substrate only
+static void supply(Microenvironment* m, int n, double* w) {
+    for (int d = 0; d < m->number_of_densities(); ++d) w[d] = n % 3 == 0 ? 0.5 * (d + 1) : 0.0;
+}
+static void target(Microenvironment* m, int, double* w) {
+    for (int d = 0; d < m->number_of_densities(); ++d) w[d] = 30.0 + d;
+}
+static void uptake(Microenvironment* m, int n, double* w) {
+    for (int d = 0; d < m->number_of_densities(); ++d) w[d] = n % 5 == 0 ? 0.2 : 0.0;
+}
 
 int main(int argc, char** argv) {
     if (argc != 3) {
@@ -83,10 +96,23 @@ int main(int argc, char** argv) {
         M.simulate_cell_sources_and_sinks(list, dt);
         for (int d = 0; d < ns; ++d) masses.push_back(M.total_mass(d));
     }
-    const std::vector<double>& out = M.density_field();
+    const std::vector<double> out = M.density_field();
+    M.compute_all_gradient_vectors();
+    std::vector<double> grad((size_t)M.number_of_voxels() * ns * 3);
+    for (int n = 0; n < M.number_of_voxels(); ++n) {
+        const double* g = M.gradient_vector(n);
+        std::copy(g, g + 3 * ns, grad.begin() + (size_t)n * ns * 3);
+    }
+    M.bulk_supply_rate_function = supply;
+    M.bulk_supply_target_densities_function = target;
+    M.bulk_uptake_rate_function = uptake;
+    M.simulate_bulk_sources_and_sinks(dt);
+    const std::vector<double>& bulk = M.density_field();
     std::ofstream o(argv[2], std::ios::binary);
     o.write(reinterpret_cast<const char*>(out.data()), out.size() * sizeof(double));
     o.write(reinterpret_cast<const char*>(masses.data()), masses.size() * sizeof(double));
+    o.write(reinterpret_cast<const char*>(grad.data()), grad.size() * sizeof(double));
+    o.write(reinterpret_cast<const char*>(bulk.data()), bulk.size() * sizeof(double));
     std::cout << "adapter_demo: " << steps << " steps of " << nx << "x" << ny << "x" << nz << "x" << ns
               << " with " << na << " agents done" << std::endl;
     return 0;

@@ -51,7 +51,11 @@ def test_cpp_host_side_matches_oracle(tmp_path, precision, tol):
     r = subprocess.run([DEMO, str(case), str(out)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     res = np.fromfile(out, np.float64)
-    got, masses = res[: field.size], res[field.size:].reshape(steps, ns)
+    nf = field.size
+    got, masses = res[:nf], res[nf: nf + steps * ns].reshape(steps, ns)
+    grad = res[nf + steps * ns: nf + steps * ns + 3 * nf].reshape(-1, ns, 3)
+    bulk = res[nf + steps * ns + 3 * nf:]
+    assert bulk.size == nf
     # oracle: same mesh, same voxel mapping, same constants
     bbox = [-half[0], -half[1], -half[2], half[0], half[1], half[2]]
     vox = oracle.voxel_indices(pos, bbox, dx, dx, dx, nx, ny, nz)
@@ -68,3 +72,20 @@ def test_cpp_host_side_matches_oracle(tmp_path, precision, tol):
         assert np.max(np.abs(masses[s] - m) / np.abs(m)) <= max(tol, 1e-12), s
     g, uu = got.reshape(-1, ns), u.reshape(-1, ns)
     assert max(relmax(g[:, d], uu[:, d]) for d in range(ns)) <= tol
+    # gradients of the final field (compute_all_gradient_vectors), and one
+    # bulk source/sink call with the demo's rate callbacks
+    gref = oracle.gradients(got, nx, ny, nz, ns, dx)
+    if precision == 0:
+        assert np.array_equal(grad, gref)
+    else:
+        assert np.max(np.abs(grad - gref)) <= 1e-9 * np.max(np.abs(gref))
+    n = np.repeat(np.arange(nx * ny * nz), ns)
+    d = np.tile(np.arange(ns), nx * ny * nz)
+    S = np.where(n % 3 == 0, 0.5 * (d + 1), 0.0)
+    T = 30.0 + d
+    U = np.where(n % 5 == 0, 0.2, 0.0)
+    bref = oracle.bulk_sources(got.copy(), ns, dt, S, T.astype(float), U)
+    if precision == 0:
+        assert np.array_equal(bulk, bref)
+    else:
+        assert relmax(bulk, bref) <= 1e-6

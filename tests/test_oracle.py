@@ -292,3 +292,73 @@ def test_voxel_index_mapping():
     edge = oracle.voxel_indices(np.array([[50.0, 40.0, 30.0], [-50.0, -40.0, -30.0], [50.1, 0, 0]]),
                                 bbox, dx, dx, dx, nx, ny, nz)
     assert list(edge) == [(4 * ny + 3) * nz + 2, 0, -1]
+
+
+# ---- gradient field (compute_all_gradient_vectors) ---------------------------------
+
+def _grid_field(nx, ny, nz, ns, f, dx=20.0):
+    """Reference-layout field with u(i,j,k,q) = f(x, y, z, q) at node coordinates."""
+    i, j, k, q = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), np.arange(ns), indexing="ij")
+    return f(i * dx, j * dx, k * dx, q).astype(float).ravel()
+
+
+def test_gradient_of_linear_field_is_exact_everywhere():
+    # central and one-sided differences are exact for linear functions; a
+    # per-substrate factor catches a substrate/component mix-up
+    nx, ny, nz, ns, dx = 6, 5, 4, 3, 20.0
+    a, b, c = 0.75, -1.5, 0.125
+    u = _grid_field(nx, ny, nz, ns, lambda x, y, z, q: (q + 1) * (a * x + b * y + c * z) + 3.0, dx)
+    g = oracle.gradients(u, nx, ny, nz, ns, dx)
+    want = np.array([[(q + 1) * a, (q + 1) * b, (q + 1) * c] for q in range(ns)])
+    assert np.max(np.abs(g - want[None])) < 1e-12
+
+
+def test_gradient_of_quadratic_field_interior_and_ends():
+    # u = x^2 + 3y^2 - z^2: the central difference is exact inside (2x); the
+    # forward difference at the low end is f'(0) + h, the backward one at the
+    # high end f'(L) - h (times the second-derivative factor of each term)
+    nx, ny, nz, ns, h = 7, 6, 5, 1, 10.0
+    u = _grid_field(nx, ny, nz, ns, lambda x, y, z, q: x * x + 3 * y * y - z * z, h)
+    g = oracle.gradients(u, nx, ny, nz, ns, h).reshape(nx, ny, nz, 3)
+    for axis, n, coef in ((0, nx, 1.0), (1, ny, 3.0), (2, nz, -1.0)):
+        comp = np.moveaxis(g[..., axis], axis, 0)
+        pos = np.arange(n) * h
+        expect = 2 * coef * pos
+        expect[0] = coef * h          # (u(h) - u(0)) / h
+        expect[-1] = coef * (2 * pos[-1] - h)  # (u(L) - u(L-h)) / h
+        assert np.allclose(comp, expect.reshape(-1, *([1] * (comp.ndim - 1))), rtol=0, atol=1e-9)
+
+
+def test_gradient_short_axes():
+    # an axis of 2 nodes: both ends take the one-sided difference; an axis
+    # of 1 node gets 0 (the reference skips z when nz == 1)
+    nx, ny, nz, ns, dx = 2, 3, 1, 1, 5.0
+    u = _grid_field(nx, ny, nz, ns, lambda x, y, z, q: 4 * x + y * y, dx)
+    g = oracle.gradients(u, nx, ny, nz, ns, dx).reshape(nx, ny, 3)
+    assert np.allclose(g[:, :, 0], 4.0)
+    assert np.allclose(g[:, 1, 1], 2 * 5.0)                    # central: 2y
+    assert np.allclose(g[:, 0, 1], 5.0) and np.allclose(g[:, 2, 1], 2 * 10.0 - 5.0)
+    assert np.all(g[:, :, 2] == 0.0)
+
+
+# ---- bulk sources and sinks (simulate_bulk_sources_and_sinks) -----------------------
+
+def test_bulk_sources_fixed_point_and_implicit_residual():
+    rng = np.random.default_rng(11)
+    nvox, ns, dt = 500, 3, 0.05
+    S = rng.uniform(0, 5, nvox * ns)
+    T = rng.uniform(0, 40, nvox * ns)
+    U = rng.uniform(0, 5, nvox * ns)
+    u0 = rng.uniform(0, 40, nvox * ns)
+    u = oracle.bulk_sources(u0.copy(), ns, dt, S, T, U)
+    # backward Euler of du/dt = S (T - u) - U u
+    resid = (u - u0) / dt - (S * (T - u) - U * u)
+    assert np.max(np.abs(resid)) < 1e-9
+    # the equilibrium S T / (S + U) is a fixed point
+    star = S * T / (S + U)
+    v = oracle.bulk_sources(star.copy(), ns, dt, S, T, U)
+    assert np.max(np.abs(v - star) / star) < 1e-14
+    # no rates: nothing changes
+    z = np.zeros_like(S)
+    w = oracle.bulk_sources(u0.copy(), ns, dt, z, T, z)
+    assert np.array_equal(w, u0)
