@@ -382,13 +382,14 @@ __global__ __launch_bounds__(256) void k_gradient_at(const T* __restrict__ u, lo
 // UpS = U + S precomputed in FP64 on the host, in the operation order of
 // simulate_bulk_sources_and_sinks (BioFVM/BioFVM_microenvironment.cpp:1327-1368):
 // one fused multiply-add for rho += dt*t2, separate roundings for t3*dt and
-// t3 + 1 (no contraction).  FP32 fields lift v to u = b + s*v in FP64 and
+// t3 + 1 (floating-point contraction is switched off in the kernel).  FP32 fields lift v to u = b + s*v in FP64 and
 // store (u - b)/s.  Grid: (blocks, ns).
 template <typename T>
 __global__ __launch_bounds__(256) void k_bulk(T* __restrict__ u, long long N, int ns,
                                               const double* __restrict__ params,
                                               const double* __restrict__ st,
                                               const double* __restrict__ ups, double dt) {
+#pragma clang fp contract(off)
     const int d = blockIdx.y;
     const bool dev = sizeof(T) == 4;
     const double b = dev ? params[2 * ns + d] : 0.0;
@@ -401,7 +402,7 @@ __global__ __launch_bounds__(256) void k_bulk(T* __restrict__ u, long long N, in
          v += (long long)gridDim.x * blockDim.x) {
         const double x = dev ? fma((double)ud[v], s, b) : (double)ud[v];
         const double rho = fma(dt, std_[v], x);
-        const double den = __dadd_rn(__dmul_rn(upd[v], dt), 1.0);
+        const double den = upd[v] * dt + 1.0;  // two roundings: contraction is off here
         const double y = rho / den;
         ud[v] = dev ? (T)((y - b) * rs) : (T)y;
     }

@@ -78,7 +78,10 @@ def test_cpp_host_side_matches_oracle(tmp_path, precision, tol):
     if precision == 0:
         assert np.array_equal(grad, gref)
     else:
-        assert np.max(np.abs(grad - gref)) <= 1e-9 * np.max(np.abs(gref))
+        # reading the field through density_field() marks the host copy as
+        # written, so the FP32 device field is re-quantised on the next upload:
+        # the gradients then agree to a few FP32 units of the field over dx
+        assert np.max(np.abs(grad - gref)) <= 4 * 2.0 ** -24 * np.max(np.abs(got)) / dx
     n = np.repeat(np.arange(nx * ny * nz), ns)
     d = np.tile(np.arange(ns), nx * ny * nz)
     S = np.where(n % 3 == 0, 0.5 * (d + 1), 0.0)
