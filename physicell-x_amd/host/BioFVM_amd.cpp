@@ -2,6 +2,7 @@
 #include "BioFVM_amd.h"
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <iostream>
 
@@ -364,6 +365,130 @@ double Microenvironment::total_mass(int substrate) {
     std::vector<double> m(number_of_densities());
     if (bfa_total_mass(env, m.data())) return NAN;
     return m[substrate];
+}
+
+void Microenvironment::write_to_matlab(const std::string& filename) {
+    download_if_needed();
+    const int ns = number_of_densities();
+    const unsigned rows = 4 + ns, cols = (unsigned)number_of_voxels();
+    std::vector<double> data((size_t)rows * cols);
+    for (int i = 0; i < mesh.x_nodes; ++i)
+        for (int j = 0; j < mesh.y_nodes; ++j)
+            for (int k = 0; k < mesh.z_nodes; ++k) {
+                const size_t n = (size_t)mesh.voxel_index(i, j, k);
+                double* col = data.data() + n * rows;
+                col[0] = mesh.x_coordinate(i);
+                col[1] = mesh.y_coordinate(j);
+                col[2] = mesh.z_coordinate(k);
+                col[3] = mesh.dV;
+                for (int d = 0; d < ns; ++d) col[4 + d] = host_field[n * ns + d];
+            }
+    write_matlab4(filename, rows, cols, "multiscale_microenvironment", data.data());
+}
+
+bool write_matlab4(const std::string& filename, unsigned rows, unsigned cols,
+                   const std::string& variable_name, const double* column_major) {
+    // header: type (0 = little-endian, double, full), rows, cols, imaginary
+    // flag, name length (without a terminating NUL, as the reference writes
+    // it), then the name and the data (BioFVM/BioFVM_matlab.cpp:536-580)
+    FILE* fp = std::fopen(filename.c_str(), "wb");
+    if (!fp) {
+        std::cout << "Error: could not open file " << filename << "!" << std::endl;
+        return false;
+    }
+    const uint32_t hdr[5] = {0u, rows, cols, 0u, (uint32_t)variable_name.size()};
+    bool ok = std::fwrite(hdr, sizeof hdr, 1, fp) == 1 &&
+              std::fwrite(variable_name.data(), 1, variable_name.size(), fp) == variable_name.size() &&
+              std::fwrite(column_major, sizeof(double), (size_t)rows * cols, fp) == (size_t)rows * cols;
+    ok = (std::fclose(fp) == 0) && ok;
+    if (!ok) std::cout << "Error: could not write " << filename << "!" << std::endl;
+    return ok;
+}
+
+std::vector<std::vector<double>> read_matlab(const std::string& filename) {
+    // BioFVM/BioFVM_matlab.cpp:71-270
+    std::vector<std::vector<double>> out;
+    FILE* fp = std::fopen(filename.c_str(), "rb");
+    if (!fp) {
+        std::cout << "Error: could not open file " << filename << "!" << std::endl;
+        return out;
+    }
+    uint32_t hdr[5];
+    if (std::fread(hdr, sizeof hdr, 1, fp) != 1) {
+        std::fclose(fp);
+        std::cout << "Error reading file " << filename << ": short header" << std::endl;
+        return out;
+    }
+    const uint32_t type = hdr[0], rows = hdr[1], cols = hdr[2], imag = hdr[3], namelen = hdr[4];
+    const uint32_t numeric = type / 1000, reserved = (type / 100) % 10, format = (type / 10) % 10,
+                   matrix = type % 10;
+    if (numeric != 0 || reserved != 0 || format > 5 || matrix != 0) {
+        std::fclose(fp);
+        std::cout << "Error reading file " << filename << ": I can't read this format yet!" << std::endl;
+        return out;
+    }
+    if (imag != 0) {
+        std::fclose(fp);
+        std::cout << "Error: I can't read imaginary matrices yet!" << std::endl;
+        return out;
+    }
+    std::vector<char> name(namelen);
+    static const size_t width[6] = {8, 4, 4, 2, 2, 1};
+    std::vector<unsigned char> raw((size_t)rows * cols * width[format]);
+    if ((namelen && std::fread(name.data(), 1, namelen, fp) != namelen) ||
+        (!raw.empty() && std::fread(raw.data(), 1, raw.size(), fp) != raw.size())) {
+        std::fclose(fp);
+        std::cout << "Error reading file " << filename << ": short data" << std::endl;
+        return out;
+    }
+    std::fclose(fp);
+    out.assign(rows, std::vector<double>(cols, 0.0));
+    for (size_t n = 0; n < (size_t)rows * cols; ++n) {
+        const unsigned char* p = raw.data() + n * width[format];
+        double v = 0.0;
+        switch (format) {
+            case 0: { double x; std::memcpy(&x, p, 8); v = x; break; }
+            case 1: { float x; std::memcpy(&x, p, 4); v = x; break; }
+            case 2: { int32_t x; std::memcpy(&x, p, 4); v = x; break; }
+            case 3: { int16_t x; std::memcpy(&x, p, 2); v = x; break; }
+            case 4: { uint16_t x; std::memcpy(&x, p, 2); v = x; break; }
+            default: v = *p; break;
+        }
+        out[n % rows][n / rows] = v;  // column-major
+    }
+    return out;
+}
+
+bool load_initial_conditions_from_matlab(Microenvironment& M, const std::string& filename) {
+    std::vector<std::vector<double>> mat = read_matlab(filename);
+    const int ns = M.number_of_densities();
+    if ((int)mat.size() != M.number_of_voxels()) {
+        std::cout << "ERROR : Wrong number of voxels supplied in the .mat file specifying BioFVM initial conditions."
+                  << std::endl
+                  << "\tExpected: " << M.number_of_voxels() << std::endl
+                  << "\tFound: " << mat.size() << std::endl;
+        return false;
+    }
+    if (mat.empty() || (int)mat[0].size() != ns + 3) {
+        std::cout << "ERROR : Wrong number of density values supplied in the .mat file specifying BioFVM initial conditions."
+                  << std::endl;
+        return false;
+    }
+    std::vector<int> vox(mat.size());
+    std::vector<char> seen(M.number_of_voxels(), 0);
+    for (size_t r = 0; r < mat.size(); ++r) {
+        vox[r] = M.mesh.nearest_voxel_index({mat[r][0], mat[r][1], mat[r][2]});
+        if (vox[r] < 0 || seen[vox[r]]) {
+            std::cout << "ERROR : the matlab-supplied initial conditions for BioFVM repeat the same voxel. "
+                         "Fix the .mat file and try again." << std::endl;
+            return false;
+        }
+        seen[vox[r]] = 1;
+    }
+    std::vector<double>& u = M.density_field();
+    for (size_t r = 0; r < mat.size(); ++r)
+        for (int d = 0; d < ns; ++d) u[(size_t)vox[r] * ns + d] = mat[r][3 + d];
+    return true;
 }
 
 void diffusion_decay_solver__constant_coefficients_LOD_3D_GPU(Microenvironment& M, double dt) {

@@ -22,6 +22,7 @@
 // reference does (BioFVM/BioFVM_solvers.cpp:116-121).
 #pragma once
 
+#include <cmath>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -48,6 +49,12 @@ struct Cartesian_Mesh {
     void resize(double x_start, double x_end, double y_start, double y_end, double z_start,
                 double z_end, double dx_new, double dy_new, double dz_new);
     int voxel_index(int i, int j, int k) const { return (i * y_nodes + j) * z_nodes + k; }
+    // voxel centres start + (i + 1/2) d, as resize computes them
+    // (BioFVM/BioFVM_mesh.cpp:1324-1329; the reference build contracts the
+    // product and sum into one fused multiply-add)
+    double x_coordinate(int i) const { return std::fma(i + 0.5, dx, bounding_box[0]); }
+    double y_coordinate(int j) const { return std::fma(j + 0.5, dy, bounding_box[1]); }
+    double z_coordinate(int k) const { return std::fma(k + 0.5, dz, bounding_box[2]); }
     int number_of_voxels() const { return x_nodes * y_nodes * z_nodes; }
     bool is_position_valid(double x, double y, double z) const;
     int nearest_voxel_index(const std::vector<double>& position) const;
@@ -132,6 +139,11 @@ class Microenvironment {
     void simulate_cell_sources_and_sinks(std::vector<Basic_Agent*>& agents, double dt);
     double total_mass(int substrate);
 
+    // MATLAB level-4 output of every voxel: a (4 + ns) x nvox matrix named
+    // multiscale_microenvironment with columns [x, y, z, dV, rho_0..rho_ns-1]
+    // (BioFVM/BioFVM_microenvironment.cpp:1222-1250)
+    void write_to_matlab(const std::string& filename);
+
     // device plumbing
     bfa_env* device();
     void download_if_needed();
@@ -158,6 +170,21 @@ class Microenvironment {
     std::vector<uint8_t> sig_active;
     std::vector<double> sig_c1, sig_c2, sig_e2;
 };
+
+// MATLAB level-4 files (BioFVM/BioFVM_matlab.cpp): read_matlab returns the
+// matrix as rows (output[row][col]; data types double, float, int32, int16,
+// uint16, uint8; little-endian full real matrices only, as the reference);
+// write_matlab4 writes a rows x cols double matrix from column-major data.
+// On error they print a message and return an empty matrix / false.
+std::vector<std::vector<double>> read_matlab(const std::string& filename);
+bool write_matlab4(const std::string& filename, unsigned rows, unsigned cols,
+                   const std::string& variable_name, const double* column_major);
+
+// Initial condition from a .mat file holding one row per voxel,
+// [x, y, z, rho_0, ..., rho_ns-1] (BioFVM/BioFVM_microenvironment.cpp:2341-2387).
+// The reference exits on a malformed file; this prints the same message and
+// returns false, leaving the field as it was.
+bool load_initial_conditions_from_matlab(Microenvironment& M, const std::string& filename);
 
 // The drop-in solver: assign it to Microenvironment::diffusion_decay_solver
 // (the reference assigns diffusion_decay_solver__constant_coefficients_LOD_3D
