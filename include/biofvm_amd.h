@@ -122,6 +122,20 @@ int bfa_apply_dirichlet(bfa_env *env);
 /* Total substrate per species, sum of rho*dx^3 over voxels, FP64 [ns]. */
 int bfa_total_mass(bfa_env *env, double *out);
 
+/* Agents from their raw state, with the voxel mapping, the constants and the
+ * per-voxel ordering computed on the device: replaces, for the whole agent
+ * list, Basic_Agent::update_voxel_index (BioFVM/BioFVM_basic_agent.cpp:177-186)
+ * and Basic_Agent::set_internal_uptake_constants (:224-274) followed by
+ * bfa_set_agents.  position [n][3]; bbox {xmin, ymin, zmin, xmax, ymax, zmax}
+ * of the mesh; volume [n]; secretion, uptake, export_rate (net export) and
+ * saturation [n][ns]; active [n] or NULL (all active).  An agent outside the
+ * box takes no part, as update_voxel_index deactivates it.  dt is the step the
+ * constants are for. */
+int bfa_set_agents_state(bfa_env *env, int n, const double *position, const double *bbox,
+                         const double *volume, const double *secretion, const double *uptake,
+                         const double *export_rate, const double *saturation,
+                         const uint8_t *active, double dt);
+
 /* Bulk sources and sinks, Microenvironment::simulate_bulk_sources_and_sinks
  * (BioFVM/BioFVM_microenvironment.cpp:1327-1368).  set_bulk_rates stores the
  * per-value supply rate S, supply target T and uptake rate U (what the

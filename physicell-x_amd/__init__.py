@@ -105,6 +105,7 @@ def lib():
     L.bfa_batch_total_mass.argtypes = [_vp, ctypes.c_int, _dp]
     L.bfa_batch_set_tiles.argtypes = [_vp] + [ctypes.c_int] * 5
     L.bfa_batch_get_tiles.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
+    L.bfa_set_agents_state.argtypes = [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _u8, ctypes.c_double]
     L.bfa_set_bulk_rates.argtypes = [_vp, _dp, _dp, _dp, ctypes.c_int]
     L.bfa_bulk_sources_and_sinks.argtypes = [_vp, ctypes.c_double]
     L.bfa_time_gradient.argtypes = [_vp, ctypes.c_int, _dp]
@@ -246,6 +247,25 @@ class Microenvironment:
 
     def slab(self):
         return lib().bfa_get_slab(self._h)
+
+    def set_agents_state(self, position, bbox, volume, secretion, uptake, export_rate, saturation,
+                         dt, active=None):
+        """Agents from their raw state (update_voxel_index and
+        set_internal_uptake_constants computed on the device)."""
+        pos = _f64(position).reshape(-1, 3)
+        n = pos.shape[0]
+        ns = self.ns
+        args = [_f64(a, n * ns) for a in (secretion, uptake, export_rate, saturation)]
+        vol = _f64(volume, n)
+        box = _f64(bbox, 6)
+        act = None
+        if active is not None:
+            act = np.ascontiguousarray(np.asarray(active, np.uint8).ravel())
+            if act.size != n:
+                raise BioFVMError(f"expected {n} activity flags, got {act.size}")
+        self._check(lib().bfa_set_agents_state(self._h, n, _d(pos), _d(box), _d(vol), *(_d(a) for a in args),
+                                               act.ctypes.data_as(_u8) if act is not None else None,
+                                               float(dt)), "set_agents_state")
 
     def set_bulk_rates(self, S, T, U, layout="ref"):
         """Per-value supply rate S, supply target T, uptake rate U for

@@ -6,6 +6,7 @@
 // face table, the agents sorted into per-voxel segments, a HIP stream and a
 // captured one-step HIP graph.  See biofvm_kernels.h for the sweep design.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -1464,3 +1465,4 @@ int bfa_get_tiles(const bfa_env* e, int* out6) {
 
 // ---- batches of independent microenvironments ------------------------------
 #include "biofvm_batch_host.inc"
+#include "biofvm_agents_dev.inc"
