@@ -265,7 +265,7 @@ def serving_requests(n_req, seed, rate=200.0, edge=(48, 256), ns_max=8, steps=(1
     return reqs, offs
 
 
-def run_serving(pkg, n_req=40, seed=5, rate=200.0, budget=200e9, k=10, spot=2):
+def run_serving(pkg, n_req=40, seed=5, rate=200.0, budget=200e9, k=10, spot=2, tiles=None, sequential=True):
     import importlib
     S = importlib.import_module("physicellx_amd.serving")
     reqs, offs = serving_requests(n_req, seed, rate)
@@ -280,7 +280,7 @@ def run_serving(pkg, n_req=40, seed=5, rate=200.0, budget=200e9, k=10, spot=2):
     sizes = [r["nx"] * r["ny"] * r["nz"] * r["ns"] for r in reqs]
     spot_ids = list(np.argsort(sizes)[:spot])
     jobs = [job(r, i in spot_ids) for i, r in enumerate(reqs)]
-    srv = S.Server(pkg, "fp32", hbm_budget=budget, steps_per_iteration=k, max_edge=256)
+    srv = S.Server(pkg, "fp32", hbm_budget=budget, steps_per_iteration=k, max_edge=256, tiles=tiles)
     wall, done = S.run_arrivals(srv, jobs, offs)
     srv.close()
     updates = sum(j.updates for j in done)
@@ -302,6 +302,8 @@ def run_serving(pkg, n_req=40, seed=5, rate=200.0, budget=200e9, k=10, spot=2):
             ref = b.download(mid)
         errs.append(float(np.max(np.abs(j.result - ref)) / np.max(np.abs(ref))))
     res["fp64_spot_check_max_rel_err"] = max(errs) if errs else None
+    if not sequential:
+        return res
     # sequential plain solves of a sample of the same requests (throughput)
     sample = [reqs[i] for i in np.random.default_rng(seed + 1).choice(n_req, min(4, n_req), replace=False)]
     tot_u, tot_t = 0, 0.0

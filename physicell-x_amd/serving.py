@@ -80,7 +80,7 @@ class Server:
     """
 
     def __init__(self, pkg, precision="fp32", hbm_budget=200e9, steps_per_iteration=10,
-                 max_edge=256, dt=0.01, clock=time.perf_counter):
+                 max_edge=256, dt=0.01, clock=time.perf_counter, tiles=None):
         self.pkg = pkg
         self.precision = precision
         self.batch = pkg.Batch(precision)
@@ -92,12 +92,13 @@ class Server:
         # shapes that cover max_edge: x rows in <= 64 lanes; y/z lines in
         # chunks of 16 nodes, as many columns per tile (<= 64) as the block
         # holds (1024 threads, 512 for chunks of 128 bytes or more)
+        # (tiles: an explicit (Lx, Ly, Lz, cols_y, cols_z) instead)
         lx = 8 if max_edge <= 512 else 16
         lyz = 16 if max_edge <= 512 else 32
         limit = 512 if lyz * (4 if precision == "fp32" else 8) >= 128 else 1024
         p = -(-max_edge // lyz)
         cols = next((c for c in (64, 32, 16) if c * p <= limit), 16)
-        self.batch.set_tiles(lx, lyz, lyz, cols, cols)
+        self.batch.set_tiles(*(tiles if tiles is not None else (lx, lyz, lyz, cols, cols)))
         self.queue = collections.deque()
         self.active = {}
         self.done = []
