@@ -86,6 +86,7 @@ def make_env(pkg, w, n, ns, precision, tiles=None):
     if tiles is not None:
         env.set_tiles(tiles["Lx"], tiles["Ly"], tiles["Lz"], tiles["rows_x"], tiles["cols_y"],
                       tiles["cols_z"])
+        env.set_x_grid(tiles.get("x_grid", 0))
         env.set_slab(tiles.get("slab", 0))
         env.set_substrate_streams(tiles.get("substrate_streams", 1))
     env.set_coefficients(w["D"], w["decay"])
@@ -432,7 +433,7 @@ def main():
     ap.add_argument("--mode", default="default", choices=["default", "single", "ensemble", "serve", "large"])
     ap.add_argument("--requests", type=int, default=40)
     ap.add_argument("--no-seq", action="store_true", help="ensemble mode: skip the sequential plain solves")
-    ap.add_argument("--tiles", default="", help="fixed FP32 shapes Lx,Ly,Lz,rows_x,cols_y,cols_z[,slab[,streams]] "
+    ap.add_argument("--tiles", default="", help="fixed FP32 shapes Lx,Ly,Lz,rows_x,cols_y,cols_z[,slab[,streams[,x_grid]]] "
                     "instead of autotuning (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -458,7 +459,8 @@ def main():
     if args.tiles:
         v = [int(x) for x in args.tiles.split(",")]
         tiles["fp32"] = dict(Lx=v[0], Ly=v[1], Lz=v[2], rows_x=v[3], cols_y=v[4], cols_z=v[5],
-                             slab=v[6] if len(v) > 6 else 0, substrate_streams=v[7] if len(v) > 7 else 1)
+                             slab=v[6] if len(v) > 6 else 0, substrate_streams=v[7] if len(v) > 7 else 1,
+                             x_grid=v[8] if len(v) > 8 else 0)
     elif not args.no_tune:
         import importlib
         at = importlib.import_module("physicellx_amd.autotune")
@@ -527,6 +529,7 @@ def main():
         "tiles": dict(Lx=tiles32[0], Ly=tiles32[1], Lz=tiles32[2], rows_x=tiles32[3],
                       cols_y=tiles32[4], cols_z=tiles32[5], slab=slab32,
                       substrate_streams=(tiles["fp32"] or {}).get("substrate_streams", 1),
+                      x_grid=(tiles["fp32"] or {}).get("x_grid", 0),
                       autotuned=not args.no_tune),
         "aux_kernels": aux,
     }

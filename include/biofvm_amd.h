@@ -210,6 +210,16 @@ int bfa_get_slab(const bfa_env *env);
 int bfa_set_substrate_streams(bfa_env *env, int groups);
 int bfa_get_substrate_streams(const bfa_env *env);
 
+/* Grid of the wave-local x sweep: blocks_per_cu 256-thread blocks per compute
+ * unit (capped at how many fit), each wavefront striding over row groups; 0
+ * (the default) launches as many as fit.  Fewer resident waves can stream
+ * faster (at 400^3 x 4 FP32, 4 blocks per CU instead of the 5 that fit);
+ * the autotuner measures it.  bfa_get_x_grid reports the requested value
+ * and how many fit for the current shape.  No reference counterpart (a
+ * launch parameter of the x sweep, BioFVM/BioFVM_solvers.cpp:352-426). */
+int bfa_set_x_grid(bfa_env *env, int blocks_per_cu);
+int bfa_get_x_grid(const bfa_env *env, int *requested, int *resident);
+
 /* ---- batches of independent microenvironments -------------------------------
  *
  * A batch advances many microenvironments of different shapes and substrate

@@ -113,6 +113,8 @@ def lib():
     L.bfa_gradients_at.argtypes = [_vp, ctypes.c_int, _i32, _dp]
     L.bfa_set_substrate_streams.argtypes = [_vp, ctypes.c_int]
     L.bfa_get_substrate_streams.argtypes = [_vp]
+    L.bfa_set_x_grid.argtypes = [_vp, ctypes.c_int]
+    L.bfa_get_x_grid.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.bfa_batch_set_fusion.argtypes = [_vp, ctypes.c_int]
     L.bfa_batch_fused_members.argtypes = [_vp]
     _lib = L
@@ -310,6 +312,16 @@ class Microenvironment:
 
     def substrate_streams(self):
         return lib().bfa_get_substrate_streams(self._h)
+
+    def set_x_grid(self, blocks_per_cu):
+        """Blocks per compute unit of the wave-local x sweep (0: as many as fit)."""
+        self._check(lib().bfa_set_x_grid(self._h, int(blocks_per_cu)), "set_x_grid")
+
+    def x_grid(self):
+        """(requested blocks per CU, blocks per CU that fit the current x shape)."""
+        req, res = ctypes.c_int(), ctypes.c_int()
+        self._check(lib().bfa_get_x_grid(self._h, ctypes.byref(req), ctypes.byref(res)), "x_grid")
+        return req.value, res.value
 
     # -- field transfer -----------------------------------------------------------
     def upload(self, field, layout="ref"):

@@ -6,7 +6,9 @@ sweep kernels on the GPU (HIP events around each launch, via
 
 * x sweep: the wave-local form with chunk length L in {4, 8, 16, 32}
   (a row's ceil(nx/L) <= 64 chunks share one wavefront), or the LDS-tiled
-  form with R rows per tile and chunk length L;
+  form with R rows per tile and chunk length L; for the wave-local form, the
+  number of resident blocks per compute unit (fewer waves in flight can
+  stream faster);
 * y and z sweeps: TX in {16, 32, 64} x-columns per block and chunk length L,
   with TX * ceil(n/L) <= 1024 threads, chosen per axis;
 * schedule: whole-volume sweeps, the two substrate halves swept on parallel
@@ -28,6 +30,7 @@ X_LDS = [(R, L) for R in (8, 16, 32) for L in (8, 16, 32)]
 COL_TX = (16, 32, 64)
 COL_L = (4, 8, 16, 32)
 SLAB_BYTES = (8 << 20, 16 << 20, 32 << 20, 64 << 20, 128 << 20)
+X_GRID = (2, 3, 4, 5, 6, 8)
 
 _cache: dict = {}
 
@@ -60,7 +63,8 @@ def _step_ms(env, dt, reps):
 
 def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=3, cache_path=None,
              verbose=False):
-    """Return {"Lx", "rows_x", "Ly", "cols_y", "Lz", "cols_z", "slab", "ms", "trials"}."""
+    """Return {"Lx", "rows_x", "Ly", "cols_y", "Lz", "cols_z", "x_grid", "slab",
+    "substrate_streams", "ms", "trials"}."""
     key = _key(nx, ny, nz, ns, precision)
     if key in _cache:
         return _cache[key]
@@ -105,6 +109,20 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=3, cache_path=
                     best_z = (ms[2], L, TX)
         shape = (best_x[1], best_y[1], best_z[1], best_x[2], best_y[2], best_z[2])
         env.set_tiles(*shape)
+        # blocks per CU of the wave-local x sweep (0: as many as fit)
+        best_grid = (best_x[0], 0)
+        if best_x[2] == 0:
+            resident = env.x_grid()[1]
+            for g in X_GRID:
+                if g >= resident:
+                    break
+                env.set_x_grid(g)
+                env.time_kernels(dt, 1)
+                ms = env.time_kernels(dt, reps)
+                trials.append({"axis": "x", "form": "wave", "L": best_x[1], "blocks_per_cu": g, "ms": ms[0]})
+                if ms[0] < best_grid[0]:
+                    best_grid = (ms[0], g)
+            env.set_x_grid(best_grid[1])
         # schedules, by graph-replayed step time: whole-volume sweeps, the two
         # substrate halves on parallel streams, x/y k-slabs
         cands = [(0, 1)]
@@ -124,9 +142,9 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=3, cache_path=
         env.set_slab(0)
         env.set_substrate_streams(1)
         res = {"Lx": best_x[1], "rows_x": best_x[2], "Ly": best_y[1], "cols_y": best_y[2],
-               "Lz": best_z[1], "cols_z": best_z[2], "slab": best_sched[1],
+               "Lz": best_z[1], "cols_z": best_z[2], "x_grid": best_grid[1], "slab": best_sched[1],
                "substrate_streams": best_sched[2],
-               "ms": {"x": best_x[0], "y": best_y[0], "z": best_z[0], "step": best_sched[0]},
+               "ms": {"x": best_grid[0], "y": best_y[0], "z": best_z[0], "step": best_sched[0]},
                "trials": trials}
     finally:
         env.close()
@@ -149,6 +167,7 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=3, cache_path=
 
 def apply(env, res):
     env.set_tiles(res["Lx"], res["Ly"], res["Lz"], res["rows_x"], res["cols_y"], res["cols_z"])
+    env.set_x_grid(res.get("x_grid", 0))
     env.set_slab(res.get("slab", 0))
     env.set_substrate_streams(res.get("substrate_streams", 1))
 

@@ -46,6 +46,7 @@ struct bfa_env {
     int max_lds = 65536;
     int cus = 256;  // compute units of the device
     int xw_occ = 1;  // resident blocks per CU of the chosen wave x-sweep kernel
+    int xw_cap = 0;  // blocks per CU the wave x sweep launches (0: xw_occ)
     double* bulk_st = nullptr;   // bulk sources: T*S per value (device order), FP64
     double* bulk_ups = nullptr;  // bulk sources: U + S per value
     hipStream_t stream = nullptr;
@@ -335,7 +336,8 @@ int launch_xw_L(bfa_env* e, hipStream_t st, const Win& w) {
     const long long groups = (w.count + RW - 1) / RW;
     // one resident wave of blocks fills the device; each wavefront then
     // strides over its row groups with the next one's loads in flight
-    const long long want = std::max<long long>(1, (long long)e->cus * e->xw_occ / w.nd);
+    const int occ = e->xw_cap > 0 ? std::min(e->xw_cap, e->xw_occ) : e->xw_occ;
+    const long long want = std::max<long long>(1, (long long)e->cus * occ / w.nd);
     const unsigned bx = (unsigned)std::max<long long>(1, std::min<long long>((groups + 3) / 4, want));
     bfa::SweepArgs<T> a = sweep_args<T>(e, 0);
     a.d0 = w.d0;
@@ -1389,6 +1391,22 @@ int bfa_set_substrate_streams(bfa_env* e, int groups) {
 }
 
 int bfa_get_substrate_streams(const bfa_env* e) { return e ? e->split : -1; }
+
+int bfa_set_x_grid(bfa_env* e, int blocks_per_cu) {
+    if (!e) return BFA_ERR_ARG;
+    if (blocks_per_cu < 0 || blocks_per_cu > 64)
+        return fail(e, BFA_ERR_ARG, "x-sweep blocks per CU must be in [0, 64], got %d", blocks_per_cu);
+    e->xw_cap = blocks_per_cu;
+    invalidate_graph(e);
+    return BFA_OK;
+}
+
+int bfa_get_x_grid(const bfa_env* e, int* requested, int* resident) {
+    if (!e) return BFA_ERR_ARG;
+    if (requested) *requested = e->xw_cap;
+    if (resident) *resident = e->xw_occ;
+    return BFA_OK;
+}
 
 int bfa_copy_kernel(size_t bytes, int reps, double* ms) {
     if (!ms || reps < 1 || bytes < 16) return BFA_ERR_ARG;

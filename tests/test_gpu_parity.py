@@ -303,6 +303,35 @@ def test_slab_schedule_is_bit_identical(pkg, precision, planes):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_x_grid_is_bit_identical(pkg, precision):
+    """Any number of blocks per CU for the wave-local x sweep gives exactly the
+    default result (the grid only changes which wave solves which row);
+    out-of-range values are rejected and leave the setting unchanged."""
+    p = make_problem(57, 33, 29, 3, seed=47, n_agents=120, dirichlet="mixed")
+    ref = gpu_env(pkg, p, precision)
+    try:
+        ref.step(p.dt, 2)
+        r = ref.download()
+        req, resident = ref.x_grid()
+        assert req == 0 and resident >= 1
+        for g in (1, 2, 3, resident + 2):
+            env = gpu_env(pkg, p, precision)
+            try:
+                env.set_x_grid(g)
+                assert env.x_grid()[0] == g
+                for bad in (-1, 65):
+                    with pytest.raises(pkg.BioFVMError):
+                        env.set_x_grid(bad)
+                assert env.x_grid()[0] == g
+                env.step(p.dt, 2)
+                assert np.array_equal(env.download(), r), g
+            finally:
+                env.close()
+    finally:
+        ref.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("ns", [2, 3])
 def test_substrate_stream_schedule_is_bit_identical(pkg, precision, ns):
     """The two substrate halves swept on parallel streams give exactly the
