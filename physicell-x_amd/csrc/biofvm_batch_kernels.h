@@ -87,6 +87,8 @@ __global__ __launch_bounds__(256) void kb_sweep_x(const EnvDesc<T>* __restrict__
                                                   int* __restrict__ ticket, int step) {
     const int lane = threadIdx.x & 63;
     const int n_items = *nitems;
+    __shared__ __align__(16) T s_stage[4 * 64 * L];  // one row-group buffer per wave
+    T* st = s_stage + (threadIdx.x >> 6) * 64 * L;
     int nxt = 0;
     if (lane == 0) nxt = atomicAdd(ticket, 1);
     for (;;) {
@@ -107,7 +109,7 @@ __global__ __launch_bounds__(256) void kb_sweep_x(const EnvDesc<T>* __restrict__
         const int valid = lane_on ? (nx - s < L ? nx - s : L) : 0;
         XCoef<T, L> c;
         x_coef_setup<T, L, DEV>(a, w.d, p, P, s, valid, c);
-        x_rows_pipelined<T, L, DEV>(a, w.d, w.row0, w.row0 + w.nrows, RW, rw, lane_on, p, P, s, valid, c);
+        x_rows_pipelined<T, L, DEV>(a, w.d, w.row0, w.row0 + w.nrows, RW, rw, lane_on, p, P, s, valid, c, st);
     }
 }
 

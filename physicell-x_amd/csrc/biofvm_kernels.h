@@ -260,9 +260,11 @@ __global__ __launch_bounds__(1024) void k_sweep_x(SweepArgs<T> a, int R, int log
 // combined with a Hillis-Steele scan of affine maps y -> a*y + b over the
 // lanes of the row (shuffles, log2(P) steps each way), and rows are
 // prefetched one group ahead so that loads of group g+1 overlap the solve of
-// group g.  The lane's 1/den and c' values stay in registers for the whole
-// kernel.  Grid: (blocks, ns) with 256-thread blocks; each wave walks row
-// groups g = wave_id, wave_id + total_waves, ...
+// group g.  A solved group is stored through a per-wave LDS buffer, so each
+// store instruction writes 64 consecutive vectors (x_row_solve_staged).  The
+// lane's 1/den and c' values stay in registers for the whole kernel.  Grid:
+// (blocks, ns) with 256-thread blocks; each wave walks row groups
+// g = wave_id, wave_id + total_waves, ...
 // 16-byte vectors with only the element's alignment: a row chunk starts at
 // an arbitrary element, and gfx950 takes unaligned global vector accesses.
 typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
@@ -279,6 +281,19 @@ struct Vec16<double> {
     using type = f64x2u;
     static constexpr int n = 2;
 };
+// the same vectors at their natural 16-byte alignment (LDS staging)
+typedef float f32x4a __attribute__((ext_vector_type(4)));
+typedef double f64x2a __attribute__((ext_vector_type(2)));
+template <typename T>
+struct Vec16A;
+template <>
+struct Vec16A<float> {
+    using type = f32x4a;
+};
+template <>
+struct Vec16A<double> {
+    using type = f64x2a;
+};
 
 // Load the L nodes of a row chunk with 16-byte vector loads.  Nodes past the
 // row end are read too (the next row's, or the zeroed slack that follows
@@ -294,28 +309,6 @@ __device__ __forceinline__ void load_chunk(T (&x)[L], const T* __restrict__ src)
         const V t = __builtin_nontemporal_load(v + q);
 #pragma unroll
         for (int w = 0; w < W; ++w) x[q * W + w] = t[w];
-    }
-}
-
-// Store the first `valid` nodes of a row chunk: whole chunks with vector
-// stores, the row's last chunk node by node.
-template <typename T, int L>
-__device__ __forceinline__ void store_chunk(const T (&x)[L], T* __restrict__ dst, int valid) {
-    if (valid == L) {
-        using V = typename Vec16<T>::type;
-        constexpr int W = Vec16<T>::n;
-        V* v = reinterpret_cast<V*>(dst);
-#pragma unroll
-        for (int q = 0; q < L / W; ++q) {
-            V t;
-#pragma unroll
-            for (int w = 0; w < W; ++w) t[w] = x[q * W + w];
-            __builtin_nontemporal_store(t, v + q);
-        }
-    } else {
-#pragma unroll
-        for (int m = 0; m < L; ++m)
-            if (m < valid) dst[m] = x[m];
     }
 }
 
@@ -417,7 +410,7 @@ __device__ __forceinline__ void x_coef_setup(const SweepArgs<T>& a, int d, int p
 }
 
 // One row group.  x_row_fetch issues the row's loads (the field chunk, and
-// the per-row Dirichlet values of a row on a y or z face); x_row_solve
+// the per-row Dirichlet values of a row on a y or z face); x_row_solve_staged
 // consumes them, so a caller can fetch the next row group before solving the
 // current one.  Dirichlet passes D0 (before) and D1 (after the solve) set
 // the row's end nodes, and all its nodes when the row lies on a y or z face.
@@ -530,12 +523,62 @@ __device__ __forceinline__ void x_row_compute(int nx, bool have, int p, int P, i
     if (have) x_row_dirichlet<T, L>(cur, nx, s, have, c, q);
 }
 
+// Store of a solved row group through LDS.  A row group (nrv rows of nx
+// nodes, contiguous in the field) is written to the wave's staging buffer st
+// (64*L nodes) at the lanes' chunk positions, then read back and stored with
+// lane m of each store instruction writing vector m of the group: every
+// instruction then writes one contiguous run of 64 vectors (whole 64-byte
+// segments), where storing the lanes' own chunks writes every other 16 bytes
+// per instruction.  The staging buffer is private to the wave, and a wave's
+// LDS accesses complete in issue order, so no barrier is needed.
 template <typename T, int L, bool DEV>
-__device__ __forceinline__ void x_row_solve(const SweepArgs<T>& a, int d, int r, bool have, int p,
-                                            int P, int s, int valid, const XCoef<T, L>& c,
-                                            XRow<T, L>& q) {
-    x_row_compute<T, L, DEV>(a.nx, have, p, P, s, c, q);
-    if (have) store_chunk<T, L>(q.v, a.u + (long long)d * a.N + (long long)r * a.nx + s, valid);
+__device__ __forceinline__ void x_row_solve_staged(const SweepArgs<T>& a, int d, int rg, int nrv,
+                                                   int rw, bool have, int p, int P, int s,
+                                                   int valid, const XCoef<T, L>& c, XRow<T, L>& q,
+                                                   T* __restrict__ st) {
+    using VA = typename Vec16A<T>::type;
+    using VU = typename Vec16<T>::type;
+    constexpr int W = Vec16<T>::n;
+    const int nx = a.nx;
+    x_row_compute<T, L, DEV>(nx, have, p, P, s, c, q);
+    if (have) {
+        T* dst = st + rw * nx + s;
+        if (nx % W == 0) {  // chunk starts on a vector boundary; valid is a multiple of W
+#pragma unroll
+            for (int v = 0; v < L / W; ++v) {
+                if (v * W < valid) {
+                    VA t;
+#pragma unroll
+                    for (int w = 0; w < W; ++w) t[w] = q.v[v * W + w];
+                    *reinterpret_cast<VA*>(dst + v * W) = t;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < L; ++m)
+                if (m < valid) dst[m] = q.v[m];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int span = nrv * nx;
+    T* gdst = a.u + (long long)d * a.N + (long long)rg * nx;
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < L / W; ++i) {
+        const int m = (i * 64 + lane) * W;
+        if (m + W <= span) {
+            const VA t = *reinterpret_cast<const VA*>(st + m);
+            VU u;
+#pragma unroll
+            for (int w = 0; w < W; ++w) u[w] = t[w];
+            __builtin_nontemporal_store(u, reinterpret_cast<VU*>(gdst + m));
+        } else {
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                if (m + w < span) gdst[m + w] = st[m + w];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
 }
 
 // Rows [r0, r1) of substrate d in row groups of RW, pipelined: the next
@@ -543,7 +586,8 @@ __device__ __forceinline__ void x_row_solve(const SweepArgs<T>& a, int d, int r,
 template <typename T, int L, bool DEV>
 __device__ __forceinline__ void x_rows_pipelined(const SweepArgs<T>& a, int d, int r0, int r1,
                                                  int RW, int rw, bool lane_on, int p, int P, int s,
-                                                 int valid, const XCoef<T, L>& c) {
+                                                 int valid, const XCoef<T, L>& c,
+                                                 T* __restrict__ st) {
     XRow<T, L> nxt;
     x_row_fetch<T, L>(a, d, r0 + rw, lane_on && r0 + rw < r1, s, valid, c, nxt);
     for (int g = r0; g < r1; g += RW) {
@@ -553,7 +597,8 @@ __device__ __forceinline__ void x_rows_pipelined(const SweepArgs<T>& a, int d, i
             const int rn = r + RW;
             x_row_fetch<T, L>(a, d, rn, lane_on && rn < r1, s, valid, c, nxt);
         }
-        x_row_solve<T, L, DEV>(a, d, r, lane_on && r < r1, p, P, s, valid, c, cur);
+        const int nrv = r1 - g < RW ? r1 - g : RW;
+        x_row_solve_staged<T, L, DEV>(a, d, g, nrv, rw, lane_on && r < r1, p, P, s, valid, c, cur, st);
     }
 }
 
@@ -579,6 +624,8 @@ __global__ __launch_bounds__(256) void k_sweep_x_wave(SweepArgs<T> a) {
     const int wstride = gridDim.x * (blockDim.x >> 6);
     XCoef<T, L> c;
     x_coef_setup<T, L, DEV>(a, d, p, P, s, valid, c);
+    __shared__ __align__(16) T s_stage[4 * 64 * L];  // one row-group buffer per wave
+    T* st = s_stage + (threadIdx.x >> 6) * 64 * L;
     // two-stage pipeline: the loads of the next row group are in flight
     // while the current one is solved
     int g = bx * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -595,7 +642,9 @@ __global__ __launch_bounds__(256) void k_sweep_x_wave(SweepArgs<T> a) {
             const int rn = row0 + gn * RW + rw;
             x_row_fetch<T, L>(a, d, rn, lane_on && rn < rows, s, valid, c, nxt);
         }
-        x_row_solve<T, L, DEV>(a, d, r, lane_on && r < rows, p, P, s, valid, c, cur);
+        const int rg = row0 + g * RW;
+        const int nrv = rows - rg < RW ? rows - rg : RW;
+        x_row_solve_staged<T, L, DEV>(a, d, rg, nrv, rw, lane_on && r < rows, p, P, s, valid, c, cur, st);
     }
 }
 
