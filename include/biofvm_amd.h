@@ -266,6 +266,14 @@ int bfa_batch_total_mass(bfa_batch *batch, int member_id, double *out);
  * automatic.  get writes {Lx, Ly, Lz, cols_y, cols_z}. */
 int bfa_batch_set_tiles(bfa_batch *batch, int Lx, int Ly, int Lz, int cols_y, int cols_z);
 int bfa_batch_get_tiles(const bfa_batch *batch, int *out5);
+/* Threads per block of the y / z column kernels (a multiple of 64, at most
+ * 1024, or 512 for chunks of 128 bytes or more); an item whose lines have P
+ * chunks then runs as tiles of about threads / P columns.  0 (the default)
+ * uses cols x the chunk count of the longest lines.  Rejected (and left
+ * unchanged) when the longest lines would get fewer than 16 columns.  get
+ * writes the y and z block sizes in use.  No reference counterpart. */
+int bfa_batch_set_col_threads(bfa_batch *batch, int threads);
+int bfa_batch_get_col_threads(const bfa_batch *batch, int *out2);
 /* Fused x + y sweeps (default off): members whose k-plane fits a 128 KiB
  * on-chip buffer are swept along x and y plane by plane in LDS, reading and
  * writing the field once for both sweeps.  Same per-line arithmetic as the

@@ -116,6 +116,8 @@ def lib():
     L.bfa_set_x_grid.argtypes = [_vp, ctypes.c_int]
     L.bfa_get_x_grid.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.bfa_batch_set_fusion.argtypes = [_vp, ctypes.c_int]
+    L.bfa_batch_set_col_threads.argtypes = [_vp, ctypes.c_int]
+    L.bfa_batch_get_col_threads.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
     L.bfa_batch_fused_members.argtypes = [_vp]
     _lib = L
     return L
@@ -491,6 +493,15 @@ class Batch:
     def set_tiles(self, Lx=0, Ly=0, Lz=0, cols_y=0, cols_z=0):
         self._check(lib().bfa_batch_set_tiles(self._h, int(Lx), int(Ly), int(Lz), int(cols_y),
                                               int(cols_z)), "batch_set_tiles")
+
+    def set_col_threads(self, threads):
+        """Threads per y/z column block (multiple of 64; 0: automatic)."""
+        self._check(lib().bfa_batch_set_col_threads(self._h, int(threads)), "batch_set_col_threads")
+
+    def col_threads(self):
+        out = (ctypes.c_int * 2)()
+        self._check(lib().bfa_batch_get_col_threads(self._h, out), "batch_get_col_threads")
+        return tuple(out)
 
     def tiles(self):
         v = (ctypes.c_int * 5)()

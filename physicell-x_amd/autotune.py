@@ -175,6 +175,7 @@ def apply(env, res):
 # ---- batches ------------------------------------------------------------------
 BATCH_X_L = (8, 16)
 BATCH_COL = [(L, TX) for L in (8, 16, 32) for TX in (16, 32, 64, 128)]
+BATCH_COL_THREADS = (512, 640, 768, 896, 1024)
 
 
 def autotune_batch(batch, dt=0.01, steps=4, verbose=False):
@@ -184,8 +185,10 @@ def autotune_batch(batch, dt=0.01, steps=4, verbose=False):
     admissible (Lx, Ly = Lz, columns per tile) and keeps the fastest; shapes the
     batch rejects (lines too long for the block) are skipped.  Every trial
     advances all members by ``steps`` steps: tune before loading the state to
-    be solved, or reload it afterwards.  Returns {"tiles": (Lx, Ly, Lz, cols_y,
-    cols_z), "ms_per_step", "trials"}.
+    be solved, or reload it afterwards.  Then, for the chosen shape, times the
+    column kernels' block sizes (threads per block: wider tiles for the same
+    lines).  Returns {"tiles": (Lx, Ly, Lz, cols_y, cols_z), "col_threads",
+    "ms_per_step", "trials"}.
     """
     import time
 
@@ -212,4 +215,23 @@ def autotune_batch(batch, dt=0.01, steps=4, verbose=False):
     if best is None:
         raise RuntimeError("no admissible batch kernel shape")
     batch.set_tiles(*best[0])
-    return {"tiles": best[0], "ms_per_step": best[1], "trials": len(trials)}
+    best_t = (0, best[1])
+    for thr in BATCH_COL_THREADS:
+        try:
+            batch.set_col_threads(thr)
+        except Exception:
+            continue
+        batch.step(dt, steps)
+        batch.synchronize()
+        t0 = time.perf_counter()
+        batch.step(dt, steps)
+        batch.synchronize()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        trials.append({"tiles": list(best[0]), "col_threads": thr, "ms_per_step": ms})
+        if verbose:
+            print(json.dumps(trials[-1]), flush=True)
+        if ms < best_t[1]:
+            best_t = (thr, ms)
+    batch.set_col_threads(best_t[0])
+    return {"tiles": best[0], "col_threads": best_t[0], "ms_per_step": best_t[1],
+            "trials": len(trials)}

@@ -192,6 +192,35 @@ def test_batch_fp32_tile_shapes(pkg):
             assert per_substrate_err(b.download(ids[k]), oracle_run(p, 3), p.ns) <= TOL["fp32"]
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_col_threads_are_bit_identical(pkg, precision):
+    """The y/z column block size only changes how many columns a tile holds:
+    every admissible size gives exactly the automatic result; sizes that are
+    not a multiple of 64, above the block limit, or too small for the
+    longest lines are rejected and leave the setting unchanged."""
+    ps = problems(seed0=670)[:4]
+    ref = None
+    for thr in (0, 512, 768, 1024):
+        with pkg.Batch(precision) as b:
+            ids = [add(b, p) for p in ps]
+            b.set_tiles(*((8, 16, 16, 32, 32) if precision == "fp32" else (8, 8, 8, 32, 32)))
+            b.set_col_threads(thr)
+            used = b.col_threads()
+            if thr:
+                assert used == (thr, thr)
+            for bad in (100, 2048, 128):  # 128 < 16 columns x 10 chunks of the 160-node lines
+                with pytest.raises(pkg.BioFVMError):
+                    b.set_col_threads(bad)
+                assert b.col_threads() == used
+            b.step(ps[0].dt, 3)
+            out = [b.download(i) for i in ids]
+        if ref is None:
+            ref = out
+        else:
+            for got, want in zip(out, ref):
+                assert np.array_equal(got, want), thr
+
+
 def served_jobs(S, seed0):
     shapes = [(40, 52, 36, 2, 7), (64, 64, 64, 1, 12), (33, 90, 47, 3, 5), (100, 40, 60, 2, 9),
               (48, 48, 48, 4, 3), (70, 35, 80, 1, 15)]
