@@ -174,64 +174,51 @@ def apply(env, res):
 
 # ---- batches ------------------------------------------------------------------
 BATCH_X_L = (8, 16)
-BATCH_COL = [(L, TX) for L in (8, 16, 32) for TX in (16, 32, 64, 128)]
-BATCH_COL_THREADS = (512, 640, 768, 896, 1024)
+BATCH_COL_L = (8, 16, 32)
+BATCH_COL_THREADS = (256, 384, 512, 640, 768, 896, 1024)
 
 
 def autotune_batch(batch, dt=0.01, steps=4, verbose=False):
     """Pick the batch kernel shapes for the batch's current membership.
 
     Times ``steps`` whole batched steps (one captured graph replay) for every
-    admissible (Lx, Ly = Lz, columns per tile) and keeps the fastest; shapes the
-    batch rejects (lines too long for the block) are skipped.  Every trial
-    advances all members by ``steps`` steps: tune before loading the state to
-    be solved, or reload it afterwards.  Then, for the chosen shape, times the
-    column kernels' block sizes (threads per block: wider tiles for the same
-    lines).  Returns {"tiles": (Lx, Ly, Lz, cols_y, cols_z), "col_threads",
-    "ms_per_step", "trials"}.
+    admissible x chunk length, y/z chunk length and threads per y/z column
+    block (an item whose lines have P chunks runs as tiles of about
+    threads / P columns), and keeps the fastest; combinations the batch
+    rejects (lines too long for the block) are skipped.  Every trial advances
+    all members by ``steps`` steps: tune before loading the state to be
+    solved, or reload it afterwards.  Returns {"tiles": (Lx, Ly, Lz, cols_y,
+    cols_z), "col_threads", "ms_per_step", "trials"}.
     """
     import time
 
     trials = []
     best = None
     for lx in BATCH_X_L:
-        for L, tx in BATCH_COL:
-            shape = (lx, L, L, tx, tx)
-            try:
-                batch.set_tiles(*shape)
-            except Exception:
-                continue
-            batch.step(dt, steps)  # capture and first replay of the shape
-            batch.synchronize()
-            t0 = time.perf_counter()
-            batch.step(dt, steps)
-            batch.synchronize()
-            ms = (time.perf_counter() - t0) / steps * 1e3
-            trials.append({"tiles": list(shape), "ms_per_step": ms})
-            if verbose:
-                print(json.dumps(trials[-1]), flush=True)
-            if best is None or ms < best[1]:
-                best = (shape, ms)
+        for L in BATCH_COL_L:
+            for thr in BATCH_COL_THREADS:
+                shape = (lx, L, L, 16, 16)
+                try:
+                    batch.set_col_threads(0)
+                    batch.set_tiles(*shape)
+                    batch.set_col_threads(thr)
+                except Exception:
+                    continue
+                batch.step(dt, steps)  # capture and first replay of the shape
+                batch.synchronize()
+                t0 = time.perf_counter()
+                batch.step(dt, steps)
+                batch.synchronize()
+                ms = (time.perf_counter() - t0) / steps * 1e3
+                trials.append({"tiles": list(shape), "col_threads": thr, "ms_per_step": ms})
+                if verbose:
+                    print(json.dumps(trials[-1]), flush=True)
+                if best is None or ms < best[2]:
+                    best = (shape, thr, ms)
     if best is None:
         raise RuntimeError("no admissible batch kernel shape")
+    batch.set_col_threads(0)
     batch.set_tiles(*best[0])
-    best_t = (0, best[1])
-    for thr in BATCH_COL_THREADS:
-        try:
-            batch.set_col_threads(thr)
-        except Exception:
-            continue
-        batch.step(dt, steps)
-        batch.synchronize()
-        t0 = time.perf_counter()
-        batch.step(dt, steps)
-        batch.synchronize()
-        ms = (time.perf_counter() - t0) / steps * 1e3
-        trials.append({"tiles": list(best[0]), "col_threads": thr, "ms_per_step": ms})
-        if verbose:
-            print(json.dumps(trials[-1]), flush=True)
-        if ms < best_t[1]:
-            best_t = (thr, ms)
-    batch.set_col_threads(best_t[0])
-    return {"tiles": best[0], "col_threads": best_t[0], "ms_per_step": best_t[1],
+    batch.set_col_threads(best[1])
+    return {"tiles": best[0], "col_threads": best[1], "ms_per_step": best[2],
             "trials": len(trials)}
