@@ -32,7 +32,8 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
-@pytest.mark.parametrize("dims", [(0, 4, 4, 1), (4, 4, 2000, 1), (4, 4, 4, 0), (4, 4, 4, 65)])
+@pytest.mark.parametrize("dims", [(0, 4, 4, 1), (4, 4, 2000, 1), (1025, 4, 4, 1), (4, 1025, 4, 1),
+                                  (4, 4, 4, 0), (4, 4, 4, 65)])
 def test_create_rejects_bad_sizes_without_a_device(dims):
     pkg = load_pkg()
     L = pkg.lib()

@@ -62,6 +62,17 @@ def test_fp64_diffusion_matches_oracle(pkg, shape):
     run_compare(pkg, p, 3, "fp64", FP64_TOL)
 
 
+@pytest.mark.parametrize("shape", [(1024, 3, 2, 1), (3, 1024, 2, 1), (2, 3, 1024, 1), (5, 4, 3, 64)])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_maximum_edges_and_substrates(pkg, shape, precision):
+    """The largest supported edge on each axis (1024 voxels) and the largest
+    substrate count (64), with mixed Dirichlet faces and agents, against the
+    oracle (edges of 1025 and 65 substrates are rejected at creation, tested
+    on the CPU)."""
+    p = make_problem(*shape, seed=sum(shape) + 7, n_agents=12, dirichlet="mixed", collide=2)
+    run_compare(pkg, p, 2, precision, FP64_TOL if precision == "fp64" else FP32_TOL)
+
+
 @pytest.mark.parametrize("dirichlet", ["all", "mixed", "off"])
 def test_fp64_dirichlet_variants(pkg, dirichlet):
     p = make_problem(24, 19, 21, 3, seed=5, n_agents=50, dirichlet=dirichlet, export=20.0)
