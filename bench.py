@@ -125,7 +125,23 @@ def load_traffic(kernel_key):
         return None
 
 
-def cpu_baseline(w, n, ns, budget_s=12.0, max_steps=40):
+def host_cpu():
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count()}
+
+
+def cpu_baseline(w, n, ns, budget_s=12.0, max_steps=40, single_thread_steps=1):
+    # thread placement as the reference's own timing runs
+    # (tests/mpi/diffusion/launch_test.sh:15-16), read when OpenMP starts
+    os.environ.setdefault("OMP_PROC_BIND", "spread")
+    os.environ.setdefault("OMP_PLACES", "threads")
     sys.path.insert(0, ROOT)
     import oracle
 
@@ -147,10 +163,22 @@ def cpu_baseline(w, n, ns, budget_s=12.0, max_steps=40):
         if time.perf_counter() - t0 > budget_s:
             break
     el = time.perf_counter() - t0
+    # one thread, for the per-core rate
+    oracle.set_num_threads(1)
+    t1 = time.perf_counter()
+    for _ in range(single_thread_steps):
+        orc.diffusion_step(ref_field, 0.01)
+        orc.secretion_uptake(ref_field, ag)
+    el1 = time.perf_counter() - t1
+    oracle.set_num_threads(threads)
     return dict(value=n ** 3 * ns * steps / el, unit="voxel·substrate updates/s", cores=threads,
                 kind="port",
                 sample=f"FP64 oracle (C/OpenMP, reference loop order) on the {n}^3 x {ns} workload, "
-                       f"{steps} steps after the Thomas precompute, {el:.1f} s, {threads} threads")
+                       f"{steps} steps after the Thomas precompute, {el:.1f} s, {threads} threads",
+                single_thread={"value": n ** 3 * ns * single_thread_steps / el1,
+                               "steps": single_thread_steps, "seconds": el1},
+                host=dict(host_cpu(), omp_proc_bind=os.environ.get("OMP_PROC_BIND"),
+                          omp_places=os.environ.get("OMP_PLACES")))
 
 
 def ensemble_members(n_env, seed):
