@@ -93,6 +93,12 @@ int bfa_set_dirichlet(bfa_env *env, const uint8_t *active, const uint8_t *face_r
  * max|u-b|) unless bfa_set_fp32_basis fixed one. */
 int bfa_upload(bfa_env *env, const double *field, int layout);
 int bfa_download(bfa_env *env, double *field, int layout);
+/* Set every voxel of substrate d to values[d] on the device (no host field
+ * transfer), as an upload of that uniform field would (for BFA_FP32 the
+ * basis is b = the Dirichlet value when active, else values[d]).  The
+ * reference initialises densities per voxel from the substrates' initial
+ * conditions (BioFVM/BioFVM_microenvironment.cpp:2487-2520). */
+int bfa_fill(bfa_env *env, const double *values);
 /* Fix the FP32 basis b[ns], s[ns] (s != 0) before the next upload. */
 int bfa_set_fp32_basis(bfa_env *env, const double *b, const double *s);
 int bfa_get_fp32_basis(const bfa_env *env, double *b, double *s);

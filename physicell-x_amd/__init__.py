@@ -69,6 +69,7 @@ def lib():
     L.bfa_set_dirichlet.argtypes = [_vp, _u8, _u8, _dp]
     L.bfa_upload.argtypes = [_vp, _dp, ctypes.c_int]
     L.bfa_download.argtypes = [_vp, _dp, ctypes.c_int]
+    L.bfa_fill.argtypes = [_vp, _dp]
     L.bfa_set_fp32_basis.argtypes = [_vp, _dp, _dp]
     L.bfa_get_fp32_basis.argtypes = [_vp, _dp, _dp]
     L.bfa_set_agents.argtypes = [_vp, ctypes.c_int, _i32, _u8, _dp, _dp, _dp]
@@ -329,6 +330,11 @@ class Microenvironment:
     def upload(self, field, layout="ref"):
         f = _f64(field, self.number_of_voxels * self.ns)
         self._check(lib().bfa_upload(self._h, _d(f), _LAYOUT[layout]), "upload")
+
+    def fill(self, values):
+        """Every voxel of substrate d set to values[d], on the device."""
+        v = _f64(values, self.ns)
+        self._check(lib().bfa_fill(self._h, _d(v)), "fill")
 
     def download(self, layout="ref"):
         out = np.empty(self.number_of_voxels * self.ns)

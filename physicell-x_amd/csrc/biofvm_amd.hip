@@ -945,6 +945,69 @@ int bfa_upload(bfa_env* e, const double* field, int layout) {
     return cleanup(BFA_OK);
 }
 
+int bfa_fill(bfa_env* e, const double* values) {
+    if (!e || !values) return fail(e, BFA_ERR_ARG, "NULL argument");
+    const int ns = e->ns;
+    for (int d = 0; d < ns; ++d)
+        if (!std::isfinite(values[d])) return fail(e, BFA_ERR_ARG, "value of substrate %d is not finite", d);
+    if (e->prec == BFA_FP32 && !e->basis_fixed) {
+        for (int d = 0; d < ns; ++d) {
+            double b = values[d];
+            if (e->active[d]) {
+                b = 1.0;
+                for (int f = 5; f >= 0; --f)
+                    if (e->reg[f]) {
+                        b = e->fvals[(size_t)f * ns + d];
+                        break;
+                    }
+            }
+            e->b[d] = b;
+            const double dev = std::fabs(values[d] - b);
+            int ex = 0;
+            if (dev > 0.0 && std::isfinite(dev)) {
+                std::frexp(dev, &ex);
+                e->s[d] = std::ldexp(1.0, ex);
+            } else {
+                e->s[d] = 1.0;
+            }
+        }
+        e->pre_valid = false;  // beta and the Dirichlet table follow the basis
+    }
+    // params: D, decay, b, s, 1/s as an upload leaves them; then the values
+    std::vector<double> p(6 * (size_t)ns, 0.0);
+    for (int d = 0; d < ns; ++d) {
+        p[d] = e->coef_set ? e->D[d] : 0.0;
+        p[ns + d] = e->coef_set ? e->decay[d] : 0.0;
+        p[2 * ns + d] = e->b[d];
+        p[3 * ns + d] = e->s[d];
+        p[4 * ns + d] = 1.0 / e->s[d];
+        p[5 * ns + d] = values[d];
+    }
+    double* vals = nullptr;
+    int rc = dalloc(e, &vals, (size_t)ns * 8);
+    if (rc) return rc;
+    hipError_t st = hipMemcpyAsync(e->params, p.data(), 5 * (size_t)ns * 8, hipMemcpyHostToDevice, e->stream);
+    if (st == hipSuccess)
+        st = hipMemcpyAsync(vals, p.data() + 5 * (size_t)ns, (size_t)ns * 8, hipMemcpyHostToDevice, e->stream);
+    if (st == hipSuccess) {
+        dim3 grid((unsigned)std::min<long long>(4096, (e->N + 255) / 256), (unsigned)ns);
+        const double* bd = e->params + 2 * ns;
+        const double* rsd = e->params + 4 * ns;
+        if (e->prec == BFA_FP32)
+            hipLaunchKernelGGL(bfa::k_fill_const<float>, grid, dim3(256), 0, e->stream, (float*)e->u, e->N,
+                               vals, bd, rsd);
+        else
+            hipLaunchKernelGGL(bfa::k_fill_const<double>, grid, dim3(256), 0, e->stream, (double*)e->u,
+                               e->N, vals, bd, rsd);
+        st = hipGetLastError();
+    }
+    if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+    (void)hipStreamSynchronize(e->stream);
+    dfree(vals);
+    if (st != hipSuccess) return fail(e, BFA_ERR_HIP, "fill failed: %s", hipGetErrorString(st));
+    return BFA_OK;
+}
+
 int bfa_download(bfa_env* e, double* field, int layout) {
     if (!e || !field) return fail(e, BFA_ERR_ARG, "NULL argument");
     if (layout != BFA_LAYOUT_REF && layout != BFA_LAYOUT_SOA)

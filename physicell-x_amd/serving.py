@@ -3,16 +3,23 @@
 A job is one microenvironment (grid, substrates, coefficients, Dirichlet
 state, initial condition, agents) to be advanced a given number of steps.  The
 server keeps every admitted job's field resident in device memory between
-steps and advances all of them together through one ``Batch``: each
-iteration runs up to ``steps_per_iteration`` steps, every job taking exactly
-the steps it still needs (the batch skips a member's work once its count for
-the call is used).  Jobs that arrive while others are on the GPU wait in a
-FIFO queue and are admitted at the next iteration boundary, as soon as the
-device-memory budget has room for them; finished jobs are retired at once
-(result downloaded, memory freed), which admits the next ones.
+steps and advances all of them together: each iteration runs up to
+``steps_per_iteration`` steps, every job taking exactly the steps it still
+needs.  Jobs below ``solo_voxels`` voxels are members of one ``Batch`` (one
+persistent kernel per sweep over all of them; the batch skips a member's work
+once its count for the call is used).  Larger jobs, whose grids keep the
+single-environment kernels near their roofline, run each as its own
+``Microenvironment`` on its own stream, launched next to the batch so that
+their kernels overlap on the device.  Jobs that arrive while others are on
+the GPU wait in a FIFO queue and are admitted at the next iteration
+boundary, as soon as the device-memory budget has room for them; finished
+jobs are retired at once (result downloaded, memory freed), which admits the
+next ones.
 
-The kernel shapes are fixed when the server starts, so a job's result does
-not depend on which other jobs shared its iterations.
+Which path a job takes depends on its own size only, and the batch's kernel
+shapes are fixed when the server starts (a solo job's shapes follow from its
+grid), so a job's result does not depend on which other jobs shared its
+iterations.
 
 There is no counterpart in the reference (it advances one Microenvironment
 per process); per job the arithmetic is that of
@@ -56,6 +63,7 @@ class Job:
     finished: float = 0.0
     remaining: int = 0
     member: int = -1
+    env: object = None   # the job's own Microenvironment (solo jobs)
     bytes: int = 0
     mass: Optional[np.ndarray] = None
     result: Optional[np.ndarray] = None
@@ -75,12 +83,13 @@ class Server:
 
     ``hbm_budget`` bounds the device bytes of admitted jobs (per
     ``member_bytes``); a job larger than the whole budget is rejected at
-    submission.  ``max_edge`` fixes the kernel shapes for the largest grid
-    edge the server accepts.
+    submission.  ``max_edge`` fixes the batch kernel shapes for the largest
+    grid edge the server accepts.  Jobs of at least ``solo_voxels`` voxels run
+    as their own Microenvironment next to the batch (None: all in the batch).
     """
 
     def __init__(self, pkg, precision="fp32", hbm_budget=200e9, steps_per_iteration=10,
-                 max_edge=256, dt=0.01, clock=time.perf_counter, tiles=None):
+                 max_edge=256, dt=0.01, clock=time.perf_counter, tiles=None, solo_voxels=None):
         self.pkg = pkg
         self.precision = precision
         self.batch = pkg.Batch(precision)
@@ -99,8 +108,10 @@ class Server:
         p = -(-max_edge // lyz)
         cols = next((c for c in (64, 32, 16) if c * p <= limit), 16)
         self.batch.set_tiles(*(tiles if tiles is not None else (lx, lyz, lyz, cols, cols)))
+        self.solo_voxels = solo_voxels
         self.queue = collections.deque()
-        self.active = {}
+        self.active = {}    # batch member id -> job
+        self.solo = {}      # job id -> job (own Microenvironment)
         self.done = []
         self.rejected = []
         self.used = 0
@@ -108,6 +119,10 @@ class Server:
         self._next_id = 0
 
     def close(self):
+        for job in self.solo.values():
+            job.env.close()
+            job.env = None
+        self.solo.clear()
         self.batch.close()
 
     # -- submission and admission ------------------------------------------------
@@ -130,6 +145,26 @@ class Server:
             self.queue.append(job)
         return job.id
 
+    def is_solo(self, job):
+        return self.solo_voxels is not None and job.nx * job.ny * job.nz >= self.solo_voxels
+
+    def _admit_solo(self, job, act, reg, fv):
+        env = self.pkg.Microenvironment(job.nx, job.ny, job.nz, job.ns, job.dx, self.precision)
+        try:
+            env.set_coefficients(job.D, job.decay)
+            env.set_dirichlet(act, reg, fv)
+            if job.init_field is not None:
+                env.upload(job.init_field)
+            else:
+                env.fill(job.init_values if job.init_values is not None else np.zeros(job.ns))
+            if job.agent_voxel is not None and len(job.agent_voxel):
+                env.set_agents(job.agent_voxel, job.agent_c1, job.agent_c2, job.agent_e2)
+        except Exception:
+            env.close()
+            raise
+        job.env = env
+        self.solo[job.id] = job
+
     def _admit(self):
         admitted = 0
         while self.queue and self.used + self.queue[0].bytes <= self.budget:
@@ -138,11 +173,23 @@ class Server:
             act = job.active if job.active is not None else np.zeros(ns, np.uint8)
             reg = job.face_registered if job.face_registered is not None else np.zeros(6, np.uint8)
             fv = job.face_values if job.face_values is not None else np.ones((6, ns))
+            if self.is_solo(job):
+                try:
+                    self._admit_solo(job, act, reg, fv)
+                except self.pkg.BioFVMError:  # device allocation failed: retry later
+                    self.queue.appendleft(job)
+                    if not self.busy_on_device():
+                        raise
+                    break
+                job.admitted = self.clock()
+                self.used += job.bytes
+                admitted += 1
+                continue
             try:
                 mid = self.batch.add(job.nx, job.ny, job.nz, ns, job.dx, job.D, job.decay, act, reg, fv)
             except self.pkg.BioFVMError as exc:  # device allocation failed: retry later
                 self.queue.appendleft(job)
-                if not self.active:
+                if not self.busy_on_device():
                     raise
                 del exc
                 break
@@ -161,24 +208,49 @@ class Server:
         return admitted
 
     # -- the loop ------------------------------------------------------------------------
+    def busy_on_device(self):
+        return bool(self.active or self.solo)
+
     def busy(self):
-        return bool(self.queue or self.active)
+        return bool(self.queue or self.active or self.solo)
 
     def iterate(self):
         """Admit what fits, run one iteration of up to k steps, retire finished jobs.
         Returns the jobs finished in this iteration."""
         self._admit()
-        if not self.active:
+        if not self.busy_on_device():
             return []
-        # a fixed run length keeps one captured graph; jobs needing fewer
-        # steps stop early inside it
         k = self.k
-        for mid, job in self.active.items():
-            self.batch.set_member_steps(mid, min(k, job.remaining))
-        self.batch.step(self.dt, k)
-        self.batch.synchronize()
+        # solo jobs first (each asynchronous on its own stream), then the batch;
+        # all of them run concurrently until the synchronisations below
+        for job in self.solo.values():
+            job.env.step(self.dt, min(k, job.remaining))
+        if self.active:
+            # a fixed run length keeps one captured graph; jobs needing fewer
+            # steps stop early inside it
+            for mid, job in self.active.items():
+                self.batch.set_member_steps(mid, min(k, job.remaining))
+            self.batch.step(self.dt, k)
+        for job in self.solo.values():
+            job.env.synchronize()
+        if self.active:
+            self.batch.synchronize()
         self.iterations += 1
         finished = []
+        for jid in list(self.solo):
+            job = self.solo[jid]
+            job.remaining -= min(k, job.remaining)
+            if job.remaining == 0:
+                if job.want_field:
+                    job.result = job.env.download()
+                job.mass = job.env.total_mass() if job.want_field else None
+                job.env.close()
+                job.env = None
+                self.used -= job.bytes
+                job.finished = self.clock()
+                del self.solo[jid]
+                self.done.append(job)
+                finished.append(job)
         for mid in list(self.active):
             job = self.active[mid]
             job.remaining -= min(k, job.remaining)

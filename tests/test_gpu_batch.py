@@ -9,7 +9,7 @@ the rest of the batch).
 import numpy as np
 import pytest
 
-from helpers import agents_for, load_pkg, make_problem, oracle_for, relmax
+from helpers import agents_for, gpu_env, load_pkg, make_problem, oracle_for, relmax
 
 pytestmark = pytest.mark.gpu
 
@@ -236,6 +236,46 @@ def served_jobs(S, seed0):
         jobs.append(j)
         probs.append(p)
     return jobs, probs
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_served_solo_and_batched_jobs_equal_alone_and_oracle(pkg, precision):
+    """Jobs of at least solo_voxels voxels run as their own Microenvironment
+    beside the batch: each result equals the same job solved alone on the
+    same path (a Microenvironment for solo jobs, a batch for the others) and
+    the oracle within the precision's bar."""
+    import importlib
+    S = importlib.import_module("physicellx_amd.serving")
+    jobs, probs = served_jobs(S, 740)
+    solo = 150000
+    srv = S.Server(pkg, precision, hbm_budget=1e10, steps_per_iteration=4, max_edge=128, solo_voxels=solo)
+    try:
+        for j in jobs:
+            srv.submit(j)
+        srv.drain()
+        tiles = srv.batch.tiles()
+    finally:
+        srv.close()
+    assert len(srv.done) == len(jobs) and not srv.rejected and not srv.solo
+    n_solo = 0
+    for j, p in zip(jobs, probs):
+        if j.nx * j.ny * j.nz >= solo:
+            n_solo += 1
+            env = gpu_env(pkg, p, precision)
+            try:
+                env.step(p.dt, j.steps)
+                assert np.array_equal(env.download(), j.result), (j.nx, j.ny, j.nz, j.ns)
+            finally:
+                env.close()
+        else:
+            with pkg.Batch(precision) as alone:
+                alone.set_tiles(*tiles)
+                mid = add(alone, p)
+                alone.step(p.dt, j.steps)
+                assert np.array_equal(alone.download(mid), j.result), (j.nx, j.ny, j.nz, j.ns)
+        err = per_substrate_err(j.result, oracle_run(p, j.steps), p.ns)
+        assert err <= TOL[precision], (j.nx, j.ny, j.nz, j.ns, err)
+    assert 0 < n_solo < len(jobs)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])

@@ -73,6 +73,39 @@ def test_maximum_edges_and_substrates(pkg, shape, precision):
     run_compare(pkg, p, 2, precision, FP64_TOL if precision == "fp64" else FP32_TOL)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fill_equals_uploading_the_uniform_field(pkg, precision):
+    """bfa_fill gives the uniform field an upload of it gives, and the same
+    steps (FP64 bit for bit; FP32 within its bar, the basis being exact for
+    a fill); non-finite values are rejected."""
+    p = make_problem(37, 21, 18, 3, seed=91, n_agents=40, dirichlet="mixed")
+    vals = np.array([3.5, 17.25, 0.0])
+    nvox = p.nx * p.ny * p.nz
+    a = pkg.Microenvironment(p.nx, p.ny, p.nz, p.ns, p.dx, precision)
+    b = pkg.Microenvironment(p.nx, p.ny, p.nz, p.ns, p.dx, precision)
+    try:
+        for e in (a, b):
+            e.set_coefficients(p.D, p.decay)
+            e.set_dirichlet(p.active, p.reg, p.fv)
+            e.set_agents(p.voxel, p.c1, p.c2, p.e2, p.agent_active)
+        a.fill(vals)
+        b.upload(np.tile(vals, nvox))
+        tol = 0.0 if precision == "fp64" else 1e-7
+        assert relmax(a.download(), np.tile(vals, nvox)) <= tol
+        a.step(p.dt, 3)
+        b.step(p.dt, 3)
+        got, want = a.download(), b.download()
+        if precision == "fp64":
+            assert np.array_equal(got, want)
+        else:
+            assert relmax(got, want) <= FP32_TOL
+        with pytest.raises(pkg.BioFVMError):
+            a.fill(np.array([1.0, np.nan, 2.0]))
+    finally:
+        a.close()
+        b.close()
+
+
 @pytest.mark.parametrize("dirichlet", ["all", "mixed", "off"])
 def test_fp64_dirichlet_variants(pkg, dirichlet):
     p = make_problem(24, 19, 21, 3, seed=5, n_agents=50, dirichlet=dirichlet, export=20.0)

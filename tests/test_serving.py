@@ -79,15 +79,66 @@ class FakeBatch:
         pass
 
 
+class FakeEnv:
+    """A solo job's Microenvironment: counts the steps it is asked for."""
+
+    def __init__(self, pkg, nx, ny, nz, ns, dx, precision):
+        if pkg.fail_env:
+            raise FakeError("out of memory")
+        self.pkg, self.ns, self.nvox = pkg, ns, nx * ny * nz
+        self.steps_done, self.pending, self.closed = 0, 0, False
+        self.uploaded = None
+        pkg.envs.append(self)
+
+    def set_coefficients(self, D, decay):
+        pass
+
+    def set_dirichlet(self, act, reg, fv):
+        pass
+
+    def upload(self, field):
+        self.uploaded = np.asarray(field)
+
+    def fill(self, values):
+        self.uploaded = np.tile(np.asarray(values, dtype=np.float64), self.nvox)
+
+    def set_agents(self, *a):
+        pass
+
+    def step(self, dt, k):
+        self.pending += k   # asynchronous until synchronize
+        self.pkg.calls.append(("env_step", self.nvox, k))
+
+    def synchronize(self):
+        self.steps_done += self.pending
+        self.pending = 0
+
+    def download(self):
+        assert self.pending == 0
+        return np.full(4, float(self.steps_done))
+
+    def total_mass(self):
+        return np.zeros(self.ns)
+
+    def close(self):
+        self.closed = True
+
+
 class FakePkg:
     BioFVMError = FakeError
 
     def __init__(self):
         self.batch = None
+        self.envs = []
+        self.calls = []
+        self.fail_env = False
 
     def Batch(self, precision):
         self.batch = FakeBatch(precision)
         return self.batch
+
+    def Microenvironment(self, *a):
+        return FakeEnv(self, *a)
 
     @staticmethod
     def member_bytes(nx, ny, nz, ns, precision):
@@ -160,6 +211,50 @@ def test_allocation_failure_requeues_when_others_run():
     # with nothing else running, an allocation failure is reported
     pkg.batch.fail_add = True
     srv.submit(job(S, 4, 1, 2))
+    with pytest.raises(FakeError):
+        srv.iterate()
+
+
+def test_large_jobs_run_solo_beside_the_batch():
+    S = load_serving()
+    pkg = FakePkg()
+    srv = S.Server(pkg, hbm_budget=1e9, steps_per_iteration=4, clock=Clock(), solo_voxels=1000)
+    small = [job(S, 8, 2, 6), job(S, 9, 1, 3)]                       # 512, 729 voxels: batch
+    big = [job(S, 10, 2, 5), job(S, 12, 3, 9, init_values=[1.0, 2.0, 3.0])]  # 1000, 1728: solo
+    for j in small + big:
+        srv.submit(j)
+    srv.iterate()
+    assert pkg.batch.calls[-1] == ("step", 4, [0, 1]) and len(pkg.envs) == 2
+    assert [c for c in pkg.calls if c[0] == "env_step"] == [("env_step", 1000, 4), ("env_step", 1728, 4)]
+    # a uniform initial value is uploaded in the reference layout (substrate fastest)
+    np.testing.assert_array_equal(pkg.envs[1].uploaded[:6], [1.0, 2.0, 3.0, 1.0, 2.0, 3.0])
+    srv.drain()
+    assert [float(j.result[0]) for j in small + big] == [6.0, 3.0, 5.0, 9.0]
+    assert all(e.closed for e in pkg.envs) and srv.used == 0 and not srv.solo
+    # each solo job was asked for exactly its own number of steps
+    per_job = {}
+    for _, nvox, k in (c for c in pkg.calls if c[0] == "env_step"):
+        per_job[nvox] = per_job.get(nvox, 0) + k
+    assert per_job == {1000: 5, 1728: 9}
+
+
+def test_solo_allocation_failure_requeues_when_others_run():
+    S = load_serving()
+    pkg = FakePkg()
+    srv = S.Server(pkg, hbm_budget=1e9, steps_per_iteration=2, clock=Clock(), solo_voxels=1000)
+    a = job(S, 4, 1, 6)                 # batch
+    srv.submit(a)
+    srv.iterate()
+    pkg.fail_env = True
+    b = job(S, 10, 1, 2)                # solo, cannot be allocated yet
+    srv.submit(b)
+    srv.iterate()
+    assert list(srv.queue) == [b] and srv.used == a.bytes
+    pkg.fail_env = False
+    srv.drain()
+    assert b.remaining == 0 and float(b.result[0]) == 2.0 and a.remaining == 0
+    pkg.fail_env = True
+    srv.submit(job(S, 10, 1, 2))        # nothing else on the device: reported
     with pytest.raises(FakeError):
         srv.iterate()
 
