@@ -1,10 +1,14 @@
 // BioFVM_amd.cpp -- C++ host side over the C ABI (see BioFVM_amd.h).
 #include "BioFVM_amd.h"
 
+#include <cctype>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <iostream>
+#include <sstream>
 
 namespace BioFVM_amd {
 
@@ -114,6 +118,12 @@ Microenvironment::Microenvironment() {
 
 Microenvironment::~Microenvironment() {
     if (env) bfa_destroy(env);
+}
+
+int Microenvironment::find_density_index(const std::string& nm) const {
+    for (int d = 0; d < (int)density_names.size(); ++d)
+        if (density_names[d] == nm) return d;
+    return -1;
 }
 
 void Microenvironment::set_density(int index, const std::string& nm, const std::string& units) {
@@ -488,6 +498,112 @@ bool load_initial_conditions_from_matlab(Microenvironment& M, const std::string&
     std::vector<double>& u = M.density_field();
     for (size_t r = 0; r < mat.size(); ++r)
         for (int d = 0; d < ns; ++d) u[(size_t)vox[r] * ns + d] = mat[r][3 + d];
+    return true;
+}
+
+// The numbers of one CSV line: skip to the next character that can start a
+// number (a digit, '.', '-', 'e' or 'E'), read it with strtod, repeat.
+static std::vector<double> csv_numbers(const std::string& line) {
+    std::vector<double> out;
+    const char* s = line.c_str();
+    const size_t n = line.size();
+    size_t i = 0;
+    while (i < n) {
+        while (i < n && !std::isdigit((unsigned char)s[i]) && s[i] != '.' && s[i] != '-' && s[i] != 'e' &&
+               s[i] != 'E')
+            ++i;
+        if (i >= n) break;
+        char* end = nullptr;
+        const double v = std::strtod(s + i, &end);
+        if (end == s + i) {  // not a number after all (e.g. a lone 'e'): skip the character
+            ++i;
+            continue;
+        }
+        out.push_back(v);
+        i = (size_t)(end - s);
+    }
+    return out;
+}
+
+bool load_initial_conditions_from_csv(Microenvironment& M, const std::string& filename) {
+    std::ifstream file(filename);
+    if (!file) {
+        std::cout << "ERROR: " << filename << " not found during cell loading. Quitting." << std::endl;
+        return false;
+    }
+    const int ns = M.number_of_densities();
+    std::vector<std::string> lines;
+    for (std::string line; std::getline(file, line);) {
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        bool blank = true;
+        for (char c : line) blank = blank && std::isspace((unsigned char)c);
+        if (!blank) lines.push_back(line);
+    }
+    std::vector<int> subs;
+    size_t first = 0;
+    const bool header = !lines.empty() && (lines[0][0] == 'x' || lines[0][0] == 'X');
+    if (header) {
+        const std::string& h = lines[0];
+        if (h.size() < 5 || (h[2] != 'Y' && h[2] != 'y') || (h[4] != 'Z' && h[4] != 'z')) {
+            std::cout << "ERROR: Header row starts with x but then not y,z? What is this? Exiting now." << std::endl;
+            return false;
+        }
+        std::stringstream stream(h);
+        std::vector<std::string> names;
+        for (std::string field; std::getline(stream, field, ',');) names.push_back(field);
+        for (size_t c = 3; c < names.size(); ++c) {
+            const int d = M.find_density_index(names[c]);
+            if (d < 0) {
+                std::cout << "ERROR: Substrate " << names[c] << " not found in the BioFVM microenvironment. Exiting now."
+                          << std::endl;
+                return false;
+            }
+            subs.push_back(d);
+        }
+        first = 1;
+    }
+    std::cout << "Loading substrate initial conditions from CSV file " << filename << " ... " << std::endl;
+    std::vector<std::vector<double>> rows;
+    bool warned = false;
+    for (size_t r = first; r < lines.size(); ++r) {
+        std::vector<double> data = csv_numbers(lines[r]);
+        if (!header && subs.empty())
+            for (int c = 0; c + 3 < (int)data.size() && c < ns; ++c) subs.push_back(c);
+        if (!warned && !header && (int)data.size() != ns + 3) {
+            std::cout << "WARNING: Wrong number of density values supplied in the .csv file specifying BioFVM initial "
+                         "conditions." << std::endl
+                      << "\tExpected: " << ns << std::endl
+                      << "\tFound: " << (int)data.size() - 3 << std::endl;
+            warned = true;
+        }
+        if (data.size() < 3 + subs.size()) {
+            std::cout << "ERROR : row " << r + 1 << " of " << filename << " has " << data.size()
+                      << " values; expected x, y, z and " << subs.size() << " densities." << std::endl;
+            return false;
+        }
+        rows.push_back(std::move(data));
+    }
+    std::vector<int> vox(rows.size());
+    std::vector<char> seen(M.number_of_voxels(), 0);
+    for (size_t r = 0; r < rows.size(); ++r) {
+        vox[r] = M.mesh.nearest_voxel_index({rows[r][0], rows[r][1], rows[r][2]});
+        if (vox[r] < 0 || seen[vox[r]]) {
+            std::cout << "ERROR : the csv-supplied initial conditions for BioFVM repeat the same voxel. Fix the .csv file "
+                         "and try again." << std::endl;
+            return false;
+        }
+        seen[vox[r]] = 1;
+    }
+    if ((int)rows.size() != M.number_of_voxels()) {
+        std::cout << "ERROR : Wrong number of voxels supplied in the .csv file specifying BioFVM initial conditions."
+                  << std::endl
+                  << "\tExpected: " << M.number_of_voxels() << std::endl
+                  << "\tFound: " << rows.size() << std::endl;
+        return false;
+    }
+    std::vector<double>& u = M.density_field();
+    for (size_t r = 0; r < rows.size(); ++r)
+        for (size_t c = 0; c < subs.size(); ++c) u[(size_t)vox[r] * ns + subs[c]] = rows[r][3 + c];
     return true;
 }
 

@@ -104,6 +104,8 @@ class Microenvironment {
     void set_density(int index, const std::string& name, const std::string& units);
     void add_density(const std::string& name, const std::string& units, double D, double decay);
     int number_of_densities() const { return (int)density_names.size(); }
+    // index of the density of that name, -1 if none (BioFVM/BioFVM_microenvironment.cpp:1043)
+    int find_density_index(const std::string& name) const;
     int number_of_voxels() const { return mesh.number_of_voxels(); }
     void resize_space_uniform(double x_start, double x_end, double y_start, double y_end,
                               double z_start, double z_end, double dx);
@@ -185,6 +187,19 @@ bool write_matlab4(const std::string& filename, unsigned rows, unsigned cols,
 // The reference exits on a malformed file; this prints the same message and
 // returns false, leaving the field as it was.
 bool load_initial_conditions_from_matlab(Microenvironment& M, const std::string& filename);
+
+// Initial condition from a CSV file with one row per voxel, [x, y, z,
+// rho_...] (BioFVM/BioFVM_microenvironment.cpp:2391-2480,2806-2840).  A first
+// row "x,y,z,name,..." names the substrates of the following columns (each
+// must exist); without it the columns after x, y, z are substrates 0, 1, ...
+// Numbers are read as the reference's csv_to_vector reads them
+// (BioFVM/BioFVM_vector.cpp:360-377).  Every voxel must be given exactly once.
+// The reference exits on a malformed file; this prints the same messages and
+// returns false, leaving the field as it was.  (The reference's own no-header
+// path skips its column counter and rereads a closed stream, so it always
+// stops with the voxel-count error; this reads the columns as its comments
+// describe.)
+bool load_initial_conditions_from_csv(Microenvironment& M, const std::string& filename);
 
 // The drop-in solver: assign it to Microenvironment::diffusion_decay_solver
 // (the reference assigns diffusion_decay_solver__constant_coefficients_LOD_3D

@@ -1,6 +1,7 @@
-"""MATLAB level-4 I/O of the C++ host side (physicell-x_amd/host), driven by
-tests/cpp/io_demo (no GPU work): load_initial_conditions_from_matlab from a
-file written by scipy, write_to_matlab read back by scipy.  Checks the
+"""MATLAB level-4 and CSV I/O of the C++ host side (physicell-x_amd/host),
+driven by tests/cpp/io_demo (no GPU work): load_initial_conditions_from_matlab
+from a file written by scipy, load_initial_conditions_from_csv, write_to_matlab
+read back by scipy.  Checks the
 reference's file layout (BioFVM/BioFVM_microenvironment.cpp:1222-1250,
 2341-2387): one column per voxel [x, y, z, dV, densities] in voxel-index
 order, voxel centres start + (i + 1/2) dx, and the rejections of a malformed
@@ -69,6 +70,60 @@ def test_malformed_initial_conditions_are_rejected(tmp_path):
         r = run([nx, ny, nz, ns, dx, p, tmp_path / "o.mat"])
         assert r.returncode == 1, (name, r.stdout)
         assert "ERROR" in r.stdout
+
+
+def write_csv(path, rows, header=None):
+    with open(path, "w") as f:
+        if header is not None:
+            f.write(header + "\n")
+        for r in rows:
+            f.write(",".join(repr(float(v)) for v in r) + "\n")
+
+
+def test_csv_initial_conditions(tmp_path):
+    """load_initial_conditions_from_csv (BioFVM/BioFVM_microenvironment.cpp:
+    2391-2480): rows in any voxel order, with no header (columns are substrates
+    0..ns-1) or with a header naming a subset of the substrates in any order
+    (the others keep their values, 0 here)."""
+    nx, ny, nz, ns, dx = 5, 4, 3, 3, 7.5
+    rng = np.random.default_rng(11)
+    field = rng.uniform(-5, 40, nx * ny * nz * ns)
+    order = rng.permutation(nx * ny * nz)
+    rows = ic_matrix(nx, ny, nz, ns, dx, field, order)
+    # no header
+    p = tmp_path / "ic.csv"
+    write_csv(p, rows)
+    out = tmp_path / "o.mat"
+    r = run([nx, ny, nz, ns, dx, p, out])
+    assert r.returncode == 0, r.stdout + r.stderr
+    m = scipy.io.loadmat(str(out))["multiscale_microenvironment"]
+    assert np.array_equal(m[4:].T.ravel(), field)
+    # header naming substrates 2 and 0 (in that order): substrate 1 stays 0
+    p2 = tmp_path / "ic_named.csv"
+    write_csv(p2, rows[:, [0, 1, 2, 5, 3]], header="x,y,z,s2,s0")
+    r = run([nx, ny, nz, ns, dx, p2, out])
+    assert r.returncode == 0, r.stdout + r.stderr
+    m = scipy.io.loadmat(str(out))["multiscale_microenvironment"]
+    want = field.reshape(-1, ns).copy()
+    want[:, 1] = 0.0
+    assert np.array_equal(m[4:].T, want)
+
+
+def test_malformed_csv_initial_conditions_are_rejected(tmp_path):
+    nx, ny, nz, ns, dx = 4, 3, 2, 2, 10.0
+    field = np.arange(nx * ny * nz * ns, dtype=float)
+    good = ic_matrix(nx, ny, nz, ns, dx, field, np.arange(nx * ny * nz))
+    cases = {"rows": (good[:-1], None), "repeat": (np.vstack([good[:-1], good[:1]]), None),
+             "name": (good, "x,y,z,s0,nope"), "head": (good, "x,q,z,s0,s1"),
+             "short": (good[:, :4], "x,y,z,s0,s1")}
+    for name, (rows, header) in cases.items():
+        p = tmp_path / f"{name}.csv"
+        write_csv(p, rows, header)
+        r = run([nx, ny, nz, ns, dx, p, tmp_path / "o.mat"])
+        assert r.returncode == 1, (name, r.stdout)
+        assert "ERROR" in r.stdout, name
+    r = run([nx, ny, nz, ns, dx, tmp_path / "missing.csv", tmp_path / "o.mat"])
+    assert r.returncode == 1 and "not found" in r.stdout
 
 
 def test_read_matlab_integer_types(tmp_path):
