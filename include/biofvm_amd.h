@@ -280,6 +280,13 @@ int bfa_batch_get_tiles(const bfa_batch *batch, int *out5);
  * writes the y and z block sizes in use.  No reference counterpart. */
 int bfa_batch_set_col_threads(bfa_batch *batch, int threads);
 int bfa_batch_get_col_threads(const bfa_batch *batch, int *out2);
+/* Stream groups: with 2, alternate members (largest first) form two groups
+ * whose sweeps run on two streams inside the same captured graph, so one
+ * group's kernels fill the tails of the other's; 1 (the default) runs every
+ * member through one kernel per sweep.  Results are bit-identical either
+ * way.  No reference counterpart. */
+int bfa_batch_set_streams(bfa_batch *batch, int groups);
+int bfa_batch_get_streams(const bfa_batch *batch);
 /* Fused x + y sweeps (default off): members whose k-plane fits a 128 KiB
  * on-chip buffer are swept along x and y plane by plane in LDS, reading and
  * writing the field once for both sweeps.  Same per-line arithmetic as the

@@ -118,6 +118,8 @@ def lib():
     L.bfa_get_x_grid.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.bfa_batch_set_fusion.argtypes = [_vp, ctypes.c_int]
     L.bfa_batch_set_col_threads.argtypes = [_vp, ctypes.c_int]
+    L.bfa_batch_set_streams.argtypes = [_vp, ctypes.c_int]
+    L.bfa_batch_get_streams.argtypes = [_vp]
     L.bfa_batch_get_col_threads.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
     L.bfa_batch_fused_members.argtypes = [_vp]
     _lib = L
@@ -508,6 +510,14 @@ class Batch:
         out = (ctypes.c_int * 2)()
         self._check(lib().bfa_batch_get_col_threads(self._h, out), "batch_get_col_threads")
         return tuple(out)
+
+    def set_streams(self, groups):
+        """1: one kernel per sweep for all members; 2: two member groups swept on
+        two streams (same results)."""
+        self._check(lib().bfa_batch_set_streams(self._h, int(groups)), "batch_set_streams")
+
+    def streams(self):
+        return lib().bfa_batch_get_streams(self._h)
 
     def tiles(self):
         v = (ctypes.c_int * 5)()

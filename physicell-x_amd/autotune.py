@@ -187,8 +187,9 @@ def autotune_batch(batch, dt=0.01, steps=4, verbose=False):
     threads / P columns), and keeps the fastest; combinations the batch
     rejects (lines too long for the block) are skipped.  Every trial advances
     all members by ``steps`` steps: tune before loading the state to be
-    solved, or reload it afterwards.  Returns {"tiles": (Lx, Ly, Lz, cols_y,
-    cols_z), "col_threads", "ms_per_step", "trials"}.
+    solved, or reload it afterwards.  Then times one against two stream
+    groups for the chosen shape.  Returns {"tiles": (Lx, Ly, Lz, cols_y,
+    cols_z), "col_threads", "streams", "ms_per_step", "trials"}.
     """
     import time
 
@@ -220,5 +221,20 @@ def autotune_batch(batch, dt=0.01, steps=4, verbose=False):
     batch.set_col_threads(0)
     batch.set_tiles(*best[0])
     batch.set_col_threads(best[1])
-    return {"tiles": best[0], "col_threads": best[1], "ms_per_step": best[2],
+    # one or two stream groups for the chosen shape
+    best_g = (1, best[2])
+    batch.set_streams(2)
+    batch.step(dt, steps)
+    batch.synchronize()
+    t0 = time.perf_counter()
+    batch.step(dt, steps)
+    batch.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    trials.append({"tiles": list(best[0]), "col_threads": best[1], "streams": 2, "ms_per_step": ms})
+    if verbose:
+        print(json.dumps(trials[-1]), flush=True)
+    if ms < best_g[1]:
+        best_g = (2, ms)
+    batch.set_streams(best_g[0])
+    return {"tiles": best[0], "col_threads": best[1], "streams": best_g[0], "ms_per_step": best_g[1],
             "trials": len(trials)}

@@ -34,6 +34,7 @@ class FakeBatch:
         self.clock = clock
         self.tiles = None
         self.threads = 0
+        self.groups = 1
         self.timed = []
 
     def _limit(self, L):
@@ -54,14 +55,20 @@ class FakeBatch:
                 raise Rejected("threads")
         self.threads = thr
 
+    def set_streams(self, g):
+        if g not in (1, 2):
+            raise Rejected("groups")
+        self.groups = g
+
     def cost(self):
         lx, L = self.tiles[0], self.tiles[1]
-        # unique minimum at Lx 8, L 16, 768 threads
-        return 3.0 + 0.2 * (lx != 8) + 0.1 * abs(L - 16) / 8 + abs(self.threads - 768) / 1000.0
+        # unique minimum at Lx 8, L 16, 768 threads; two stream groups 5 % faster
+        c = 3.0 + 0.2 * (lx != 8) + 0.1 * abs(L - 16) / 8 + abs(self.threads - 768) / 1000.0
+        return c * (0.95 if self.groups == 2 else 1.0)
 
     def step(self, dt, n):
         self.clock.t += self.cost() * n * 1e-3
-        self.timed.append((self.tiles, self.threads))
+        self.timed.append((self.tiles, self.threads, self.groups))
 
     def synchronize(self):
         pass
@@ -80,10 +87,11 @@ def test_batch_autotune_picks_the_fastest_admissible_shape(monkeypatch):
     monkeypatch.setattr(time, "perf_counter", clock)
     b = FakeBatch(clock)
     res = at.autotune_batch(b, 0.01, steps=4)
-    assert res["tiles"][:3] == (8, 16, 16) and res["col_threads"] == 768
-    assert res["ms_per_step"] == pytest.approx(3.0)
-    assert b.tiles[:3] == (8, 16, 16) and b.threads == 768  # left applied
-    tried = {(t[0], t[1], thr) for t, thr in b.timed}
+    assert res["tiles"][:3] == (8, 16, 16) and res["col_threads"] == 768 and res["streams"] == 2
+    assert res["ms_per_step"] == pytest.approx(3.0 * 0.95)
+    assert b.tiles[:3] == (8, 16, 16) and b.threads == 768 and b.groups == 2  # left applied
+    tried = {(t[0], t[1], thr) for t, thr, g in b.timed if g == 1}
+    assert {(t[1], thr) for t, thr, g in b.timed if g == 2} == {(16, 768)}
     # every admissible combination was timed, no rejected one was
     for lx in at.BATCH_X_L:
         for L in at.BATCH_COL_L:
@@ -91,4 +99,4 @@ def test_batch_autotune_picks_the_fastest_admissible_shape(monkeypatch):
                 P = -(-160 // L)
                 ok = 16 * P <= thr <= (512 if L * 4 >= 128 else 1024)
                 assert ((lx, L, thr) in tried) == ok, (lx, L, thr)
-    assert res["trials"] == len(tried)
+    assert res["trials"] == len(tried) + 1

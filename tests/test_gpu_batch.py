@@ -221,6 +221,35 @@ def test_batch_col_threads_are_bit_identical(pkg, precision):
                 assert np.array_equal(got, want), thr
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_stream_groups_are_bit_identical(pkg, precision):
+    """Two member groups on two streams give exactly the one-group result,
+    also with members stopping early (per-member step counts) and after
+    switching back; other group counts are rejected."""
+    ps = problems(seed0=690)
+    outs = []
+    for groups in (1, 2, 1):
+        with pkg.Batch(precision) as b:
+            ids = [add(b, p) for p in ps]
+            b.set_streams(groups)
+            assert b.streams() == groups
+            for bad in (0, 3):
+                with pytest.raises(pkg.BioFVMError):
+                    b.set_streams(bad)
+            assert b.streams() == groups
+            b.step(ps[0].dt, 2)
+            b.set_member_steps(ids[1], 1)
+            b.set_member_steps(ids[4], 0)
+            b.step(ps[0].dt, 2)
+            outs.append([b.download(i) for i in ids])
+    for k in (1, 2):
+        for got, want in zip(outs[k], outs[0]):
+            assert np.array_equal(got, want), k
+    # and the two-group run equals the oracle for a member that ran all 4 steps
+    p = ps[0]
+    assert per_substrate_err(outs[1][0], oracle_run(p, 4), p.ns) <= TOL[precision]
+
+
 def served_jobs(S, seed0):
     shapes = [(40, 52, 36, 2, 7), (64, 64, 64, 1, 12), (33, 90, 47, 3, 5), (100, 40, 60, 2, 9),
               (48, 48, 48, 4, 3), (70, 35, 80, 1, 15)]
