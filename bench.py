@@ -241,7 +241,7 @@ def run_ensemble(pkg, n_env=256, steps=100, repeats=3, seed=4, sequential=True, 
         lat = np.array(lat)
         res[prec] = {"value": updates / float(np.median(lat)), "p50_s": float(np.percentile(lat, 50)),
                      "p99_s": float(np.percentile(lat, 99)), "runs": repeats, "tiles": list(b.tiles()),
-                     "col_threads": list(b.col_threads()),
+                     "col_threads": list(b.col_threads()), "streams": b.streams(),
                      "autotune_trials": tuned["trials"] if tuned else 0}
         b.close()
     errs = [float(np.max(np.abs(a - c)) / np.max(np.abs(c))) for a, c in zip(fields["fp32"], fields["fp64"])]
