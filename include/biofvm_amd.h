@@ -280,11 +280,11 @@ int bfa_batch_get_tiles(const bfa_batch *batch, int *out5);
  * writes the y and z block sizes in use.  No reference counterpart. */
 int bfa_batch_set_col_threads(bfa_batch *batch, int threads);
 int bfa_batch_get_col_threads(const bfa_batch *batch, int *out2);
-/* Stream groups: with 2, alternate members (largest first) form two groups
- * whose sweeps run on two streams inside the same captured graph, so one
- * group's kernels fill the tails of the other's; 1 (the default) runs every
- * member through one kernel per sweep.  Results are bit-identical either
- * way.  No reference counterpart. */
+/* Stream groups: with G in [2, 4], members (largest first) are dealt
+ * round-robin into G groups whose sweeps run on G streams inside the same
+ * captured graph, so one group's kernels fill the tails of the others'; 1
+ * (the default) runs every member through one kernel per sweep.  Results are
+ * bit-identical either way.  No reference counterpart. */
 int bfa_batch_set_streams(bfa_batch *batch, int groups);
 int bfa_batch_get_streams(const bfa_batch *batch);
 /* Fused x + y sweeps (default off): members whose k-plane fits a 128 KiB

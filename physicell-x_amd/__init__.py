@@ -512,8 +512,8 @@ class Batch:
         return tuple(out)
 
     def set_streams(self, groups):
-        """1: one kernel per sweep for all members; 2: two member groups swept on
-        two streams (same results)."""
+        """1: one kernel per sweep for all members; 2..4: member groups swept on
+        that many streams (same results)."""
         self._check(lib().bfa_batch_set_streams(self._h, int(groups)), "batch_set_streams")
 
     def streams(self):

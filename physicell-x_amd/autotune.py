@@ -176,6 +176,7 @@ def apply(env, res):
 BATCH_X_L = (8, 16)
 BATCH_COL_L = (8, 16, 32)
 BATCH_COL_THREADS = (256, 384, 512, 640, 768, 896, 1024)
+BATCH_STREAMS = (2, 3, 4)
 
 
 def autotune_batch(batch, dt=0.01, steps=4, verbose=False):
@@ -187,8 +188,8 @@ def autotune_batch(batch, dt=0.01, steps=4, verbose=False):
     threads / P columns), and keeps the fastest; combinations the batch
     rejects (lines too long for the block) are skipped.  Every trial advances
     all members by ``steps`` steps: tune before loading the state to be
-    solved, or reload it afterwards.  Then times one against two stream
-    groups for the chosen shape.  Returns {"tiles": (Lx, Ly, Lz, cols_y,
+    solved, or reload it afterwards.  Then times one to four stream groups
+    for the chosen shape.  Returns {"tiles": (Lx, Ly, Lz, cols_y,
     cols_z), "col_threads", "streams", "ms_per_step", "trials"}.
     """
     import time
@@ -221,20 +222,21 @@ def autotune_batch(batch, dt=0.01, steps=4, verbose=False):
     batch.set_col_threads(0)
     batch.set_tiles(*best[0])
     batch.set_col_threads(best[1])
-    # one or two stream groups for the chosen shape
+    # stream groups for the chosen shape
     best_g = (1, best[2])
-    batch.set_streams(2)
-    batch.step(dt, steps)
-    batch.synchronize()
-    t0 = time.perf_counter()
-    batch.step(dt, steps)
-    batch.synchronize()
-    ms = (time.perf_counter() - t0) / steps * 1e3
-    trials.append({"tiles": list(best[0]), "col_threads": best[1], "streams": 2, "ms_per_step": ms})
-    if verbose:
-        print(json.dumps(trials[-1]), flush=True)
-    if ms < best_g[1]:
-        best_g = (2, ms)
+    for groups in BATCH_STREAMS:
+        batch.set_streams(groups)
+        batch.step(dt, steps)
+        batch.synchronize()
+        t0 = time.perf_counter()
+        batch.step(dt, steps)
+        batch.synchronize()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        trials.append({"tiles": list(best[0]), "col_threads": best[1], "streams": groups, "ms_per_step": ms})
+        if verbose:
+            print(json.dumps(trials[-1]), flush=True)
+        if ms < best_g[1]:
+            best_g = (groups, ms)
     batch.set_streams(best_g[0])
     return {"tiles": best[0], "col_threads": best[1], "streams": best_g[0], "ms_per_step": best_g[1],
             "trials": len(trials)}

@@ -56,7 +56,7 @@ class FakeBatch:
         self.threads = thr
 
     def set_streams(self, g):
-        if g not in (1, 2):
+        if g not in (1, 2, 3, 4):
             raise Rejected("groups")
         self.groups = g
 
@@ -91,7 +91,7 @@ def test_batch_autotune_picks_the_fastest_admissible_shape(monkeypatch):
     assert res["ms_per_step"] == pytest.approx(3.0 * 0.95)
     assert b.tiles[:3] == (8, 16, 16) and b.threads == 768 and b.groups == 2  # left applied
     tried = {(t[0], t[1], thr) for t, thr, g in b.timed if g == 1}
-    assert {(t[1], thr) for t, thr, g in b.timed if g == 2} == {(16, 768)}
+    assert {(t[1], thr, g) for t, thr, g in b.timed if g > 1} == {(16, 768, g) for g in (2, 3, 4)}
     # every admissible combination was timed, no rejected one was
     for lx in at.BATCH_X_L:
         for L in at.BATCH_COL_L:
@@ -99,4 +99,4 @@ def test_batch_autotune_picks_the_fastest_admissible_shape(monkeypatch):
                 P = -(-160 // L)
                 ok = 16 * P <= thr <= (512 if L * 4 >= 128 else 1024)
                 assert ((lx, L, thr) in tried) == ok, (lx, L, thr)
-    assert res["trials"] == len(tried) + 1
+    assert res["trials"] == len(tried) + 3

@@ -223,17 +223,17 @@ def test_batch_col_threads_are_bit_identical(pkg, precision):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_batch_stream_groups_are_bit_identical(pkg, precision):
-    """Two member groups on two streams give exactly the one-group result,
-    also with members stopping early (per-member step counts) and after
-    switching back; other group counts are rejected."""
+    """Two to four member groups on as many streams give exactly the
+    one-group result, also with members stopping early (per-member step
+    counts) and after switching back; other group counts are rejected."""
     ps = problems(seed0=690)
     outs = []
-    for groups in (1, 2, 1):
+    for groups in (1, 2, 3, 4, 1):
         with pkg.Batch(precision) as b:
             ids = [add(b, p) for p in ps]
             b.set_streams(groups)
             assert b.streams() == groups
-            for bad in (0, 3):
+            for bad in (0, 5):
                 with pytest.raises(pkg.BioFVMError):
                     b.set_streams(bad)
             assert b.streams() == groups
@@ -242,7 +242,7 @@ def test_batch_stream_groups_are_bit_identical(pkg, precision):
             b.set_member_steps(ids[4], 0)
             b.step(ps[0].dt, 2)
             outs.append([b.download(i) for i in ids])
-    for k in (1, 2):
+    for k in range(1, len(outs)):
         for got, want in zip(outs[k], outs[0]):
             assert np.array_equal(got, want), k
     # and the two-group run equals the oracle for a member that ran all 4 steps
