@@ -114,6 +114,29 @@ def time_gpu(env, dt, steps, warmup):
     return wall, ms, eager_wall
 
 
+def run_host_boundary(env, dt, updates, reps=3):
+    """The drop-in solver form, where the caller's field stays in host memory
+    (FP64, reference layout, pageable): every step uploads it, runs one step
+    and downloads it again.  Median over reps of each part, PCIe included."""
+    host = env.download("ref")
+    parts = []
+    for _ in range(reps + 1):  # the first round pays first-touch costs
+        t0 = time.perf_counter()
+        env.upload(host, "ref")
+        env.synchronize()
+        t1 = time.perf_counter()
+        env.step(dt, 1)
+        env.synchronize()
+        t2 = time.perf_counter()
+        env.download("ref", out=host)
+        parts.append((t1 - t0, t2 - t1, time.perf_counter() - t2))
+    p = np.array(parts[1:]) * 1e3
+    total = float(np.median(p.sum(axis=1)))
+    return {"upload_ms": float(np.median(p[:, 0])), "step_ms": float(np.median(p[:, 1])),
+            "download_ms": float(np.median(p[:, 2])), "ms_per_step": total,
+            "value": updates / (total * 1e-3), "field_bytes_each_way": int(host.nbytes), "reps": reps}
+
+
 def load_traffic(kernel_key):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
@@ -530,8 +553,12 @@ def main():
         res["fp32_vs_fp64"] = dict(max_rel_err_per_substrate=errs, mass_drift_per_substrate=drift,
                                    steps=args.warmup + 2 * args.steps,
                                    norm="max|f32 - f64| / max|f64| per substrate")
+        host_boundary = {"fp64": run_host_boundary(env64, dt, updates)}
         env64.close()
+    else:
+        host_boundary = {}
     aux = run_aux(env32, w, updates)
+    host_boundary["fp32"] = run_host_boundary(env32, dt, updates)
     env32.close()
 
     slab32 = tiles["fp32"]["slab"] if tiles["fp32"] else 0
@@ -571,6 +598,8 @@ def main():
                       x_grid=(tiles["fp32"] or {}).get("x_grid", 0),
                       autotuned=not args.no_tune),
         "aux_kernels": aux,
+        "host_boundary": dict(host_boundary, note="field in host memory (FP64, reference layout): "
+                              "upload + one step + download per step, PCIe included; not the value"),
     }
     out.update(res)
     if args.mode == "default":

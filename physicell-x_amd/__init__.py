@@ -338,8 +338,14 @@ class Microenvironment:
         v = _f64(values, self.ns)
         self._check(lib().bfa_fill(self._h, _d(v)), "fill")
 
-    def download(self, layout="ref"):
-        out = np.empty(self.number_of_voxels * self.ns)
+    def download(self, layout="ref", out=None):
+        """The field as FP64 in the given layout, into `out` when given (a
+        C-contiguous float64 array of number_of_voxels * ns values)."""
+        size = self.number_of_voxels * self.ns
+        if out is None:
+            out = np.empty(size)
+        elif out.dtype != np.float64 or out.size != size or not out.flags.c_contiguous:
+            raise ValueError(f"download: out must be a C-contiguous float64 array of {size} values")
         self._check(lib().bfa_download(self._h, _d(out), _LAYOUT[layout]), "download")
         return out
 

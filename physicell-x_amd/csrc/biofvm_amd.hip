@@ -21,6 +21,7 @@
 #include "biofvm_aux_kernels.h"
 #include "biofvm_batch_kernels.h"
 #include "biofvm_kernels.h"
+#include "biofvm_transfer.inc"
 
 namespace {
 
@@ -857,8 +858,7 @@ int bfa_upload(bfa_env* e, const double* field, int layout) {
     double* stage = nullptr;
     const bool direct = (e->prec == BFA_FP64 && layout == BFA_LAYOUT_SOA);
     if (direct) {
-        HIP_TRY(e, hipMemcpyAsync(e->u, field, total * 8, hipMemcpyHostToDevice, e->stream));
-        HIP_TRY(e, hipStreamSynchronize(e->stream));
+        HIP_TRY(e, copy_h2d(e->u, field, total * 8, e->stream));
         return BFA_OK;
     }
     int rc = dalloc(e, &stage, total * 8);
@@ -868,7 +868,7 @@ int bfa_upload(bfa_env* e, const double* field, int layout) {
         dfree(stage);
         return code;
     };
-    hipError_t st = hipMemcpyAsync(stage, field, total * 8, hipMemcpyHostToDevice, e->stream);
+    hipError_t st = copy_h2d(stage, field, total * 8, e->stream);
     if (st != hipSuccess)
         return cleanup(fail(e, BFA_ERR_HIP, "upload copy failed: %s", hipGetErrorString(st)));
     // basis for the FP32 deviation form
@@ -1014,8 +1014,7 @@ int bfa_download(bfa_env* e, double* field, int layout) {
         return fail(e, BFA_ERR_ARG, "unknown layout %d", layout);
     const size_t total = (size_t)e->N * e->ns;
     if (e->prec == BFA_FP64 && layout == BFA_LAYOUT_SOA) {
-        HIP_TRY(e, hipMemcpyAsync(field, e->u, total * 8, hipMemcpyDeviceToHost, e->stream));
-        HIP_TRY(e, hipStreamSynchronize(e->stream));
+        HIP_TRY(e, copy_d2h(field, e->u, total * 8, e->stream));
         return BFA_OK;
     }
     double* stage = nullptr;
@@ -1054,8 +1053,7 @@ int bfa_download(bfa_env* e, double* field, int layout) {
         }
         st = hipGetLastError();
     }
-    if (st == hipSuccess) st = hipMemcpyAsync(field, stage, total * 8, hipMemcpyDeviceToHost, e->stream);
-    if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+    if (st == hipSuccess) st = copy_d2h(field, stage, total * 8, e->stream);
     dfree(dbs);
     if (st != hipSuccess) return cleanup(fail(e, BFA_ERR_HIP, "download failed: %s", hipGetErrorString(st)));
     return cleanup(BFA_OK);
@@ -1326,8 +1324,7 @@ int bfa_gradients(bfa_env* e, double* out, int layout) {
             return cleanup(fail(e, BFA_ERR_HIP, "gradient layout failed: %s", hipGetErrorString(st)));
         src = stage;
     }
-    hipError_t st = hipMemcpyAsync(out, src, total * 8, hipMemcpyDeviceToHost, e->stream);
-    if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+    hipError_t st = copy_d2h(out, src, total * 8, e->stream);
     if (st != hipSuccess) return cleanup(fail(e, BFA_ERR_HIP, "gradient download failed: %s", hipGetErrorString(st)));
     return cleanup(BFA_OK);
 }

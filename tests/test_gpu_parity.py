@@ -101,6 +101,15 @@ def test_fill_equals_uploading_the_uniform_field(pkg, precision):
             assert relmax(got, want) <= FP32_TOL
         with pytest.raises(pkg.BioFVMError):
             a.fill(np.array([1.0, np.nan, 2.0]))
+        # download into a caller's buffer, in both layouts
+        for layout in ("ref", "soa"):
+            buf = np.full(nvox * p.ns, -1.0)
+            assert a.download(layout, out=buf) is buf
+            assert np.array_equal(buf, a.download(layout))
+        with pytest.raises(ValueError):
+            a.download(out=np.empty(nvox * p.ns, np.float32))
+        with pytest.raises(ValueError):
+            a.download(out=np.empty(nvox * p.ns - 1))
     finally:
         a.close()
         b.close()
@@ -218,6 +227,48 @@ def test_layout_round_trips(pkg, precision):
         assert relmax(back, p.u0) < (1e-7 if precision == "fp32" else 1e-300)
     finally:
         env.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_staged_transfers_of_large_fields(pkg, precision):
+    """Fields of 16 MiB and more move through the pinned chunk pipeline
+    (64 MiB chunks): 211 x 197 x 163 x 4 FP64 values = 207 MiB, three full
+    chunks and a partial one, in both layouts, for an environment and for a
+    batch member of 160^3 x 4 (two chunks)."""
+    nx, ny, nz, ns = 211, 197, 163, 4
+    rng = np.random.default_rng(77)
+    ref = rng.uniform(1.0, 50.0, nx * ny * nz * ns)
+    soa = ref.reshape(nx, ny, nz, ns).transpose(3, 2, 1, 0).ravel()
+    tol = 0.0 if precision == "fp64" else 1e-7
+    env = pkg.Microenvironment(nx, ny, nz, ns, 20.0, precision)
+    try:
+        env.set_coefficients(np.full(ns, 1e3), np.full(ns, 0.1))
+        env.upload(ref, "ref")
+        assert relmax(env.download("soa"), soa) <= tol
+        buf = np.empty_like(ref)
+        assert relmax(env.download("ref", out=buf), ref) <= tol
+        env.upload(soa, "soa")
+        assert relmax(env.download("ref"), ref) <= tol
+        assert relmax(env.download("soa"), soa) <= tol
+        # the same upload and step give the same field again
+        env.step(0.01, 1)
+        stepped = env.download("ref")
+        env.upload(soa, "soa")
+        env.step(0.01, 1)
+        assert np.array_equal(env.download("ref"), stepped)
+    finally:
+        env.close()
+    n = 160
+    mref = rng.uniform(1.0, 50.0, n ** 3 * ns)
+    b = pkg.Batch(precision)
+    try:
+        mid = b.add(n, n, n, ns, 20.0, np.full(ns, 1e3), np.full(ns, 0.1))
+        b.upload(mid, mref, "ref")
+        assert relmax(b.download(mid, "ref"), mref) <= tol
+        msoa = mref.reshape(n, n, n, ns).transpose(3, 2, 1, 0).ravel()
+        assert relmax(b.download(mid, "soa"), msoa) <= tol
+    finally:
+        b.close()
 
 
 def test_total_mass_and_dirichlet_pass(pkg):
