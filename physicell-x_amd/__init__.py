@@ -137,6 +137,17 @@ def _f64(a, n=None):
     return a
 
 
+def _flags(a, n):
+    """Per-agent activity flags as a contiguous uint8 array of exactly n
+    entries (the C side reads n of them), or None for all active."""
+    if a is None:
+        return None
+    a = np.ascontiguousarray(np.asarray(a).ravel(), dtype=np.uint8)
+    if a.size != n:
+        raise BioFVMError(f"expected {n} activity flags, got {a.size}")
+    return a
+
+
 def copy_kernel_ms(nbytes: int, reps: int = 20) -> float:
     """Mean time of a plain device copy of nbytes (HBM counter calibration)."""
     out = ctypes.c_double()
@@ -357,7 +368,7 @@ class Microenvironment:
         c1 = _f64(c1, n * self.ns)
         c2 = _f64(c2, n * self.ns)
         e2 = _f64(e2, n * self.ns)
-        act = None if active is None else np.ascontiguousarray(active, np.uint8)
+        act = _flags(active, n)
         self._check(lib().bfa_set_agents(self._h, n, voxel.ctypes.data_as(_i32),
                                          None if act is None else act.ctypes.data_as(_u8),
                                          _d(c1), _d(c2), _d(e2)), "set_agents")
@@ -478,7 +489,7 @@ class Batch:
         ns = self.members[mid][3]
         voxel = np.ascontiguousarray(voxel, np.int32)
         n = voxel.size
-        act = None if active is None else np.ascontiguousarray(active, np.uint8)
+        act = _flags(active, n)
         self._check(lib().bfa_batch_set_agents(self._h, int(mid), n, voxel.ctypes.data_as(_i32),
                                                None if act is None else act.ctypes.data_as(_u8),
                                                _d(_f64(c1, n * ns)), _d(_f64(c2, n * ns)),

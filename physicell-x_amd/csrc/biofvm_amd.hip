@@ -526,7 +526,12 @@ int launch_agents(bfa_env* e) {
     return BFA_OK;
 }
 
+// Drops the captured step graph.  Launches of it may still be queued or
+// running (bfa_step only enqueues them), and its kernel arguments point into
+// buffers that the caller is about to replace, so the handle's stream (which
+// every side stream joins back into) is drained first.
 void invalidate_graph(bfa_env* e) {
+    if (e->gexec && e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
     e->gexec = nullptr;

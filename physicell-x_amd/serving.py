@@ -193,14 +193,21 @@ class Server:
                     raise
                 del exc
                 break
+            try:
+                if job.init_field is not None:
+                    self.batch.upload(mid, job.init_field)
+                else:
+                    vals = job.init_values if job.init_values is not None else np.zeros(ns)
+                    self.batch.fill(mid, vals)
+                if job.agent_voxel is not None and len(job.agent_voxel):
+                    self.batch.set_agents(mid, job.agent_voxel, job.agent_c1, job.agent_c2, job.agent_e2)
+            except Exception:
+                # the member was allocated but not set up: free it so its
+                # device memory does not leak outside the budget, and reject
+                # the job (its inputs are what failed)
+                self.batch.remove(mid)
+                raise
             job.member = mid
-            if job.init_field is not None:
-                self.batch.upload(mid, job.init_field)
-            else:
-                vals = job.init_values if job.init_values is not None else np.zeros(ns)
-                self.batch.fill(mid, vals)
-            if job.agent_voxel is not None and len(job.agent_voxel):
-                self.batch.set_agents(mid, job.agent_voxel, job.agent_c1, job.agent_c2, job.agent_e2)
             job.admitted = self.clock()
             self.used += job.bytes
             self.active[mid] = job

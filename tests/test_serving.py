@@ -32,6 +32,7 @@ class FakeBatch:
         self.calls = []
         self.next_id = 0
         self.fail_add = False
+        self.fail_setup = False
 
     def set_tiles(self, *a):
         self.tiles = a
@@ -53,6 +54,8 @@ class FakeBatch:
         self.calls.append(("upload", mid))
 
     def set_agents(self, *a):
+        if self.fail_setup:
+            raise FakeError("bad agent arrays")
         self.calls.append(("agents", a[0]))
 
     def set_member_steps(self, mid, s):
@@ -269,3 +272,23 @@ def test_latency_stats_and_arrivals():
     st = S.latency_stats(done)
     assert st["p50_s"] <= st["p99_s"] <= st["max_s"]
     assert sorted(float(j.result[0]) for j in done) == [2.0, 3.0, 7.0, 11.0]
+
+
+def test_member_set_up_failure_frees_the_member():
+    S = load_serving()
+    pkg = FakePkg()
+    srv = S.Server(pkg, hbm_budget=1e9, steps_per_iteration=2, clock=Clock())
+    srv.iterate()
+    pkg.batch.fail_setup = True
+    bad = job(S, 4, 1, 2, agent_voxel=np.array([0, 1], np.int32), agent_c1=np.zeros(2),
+              agent_c2=np.ones(2), agent_e2=np.zeros(2))
+    srv.submit(bad)
+    with pytest.raises(FakeError):
+        srv.iterate()
+    # the member added for it is gone and nothing is charged to the budget
+    assert not pkg.batch.members and srv.used == 0 and not srv.active
+    pkg.batch.fail_setup = False
+    ok = job(S, 4, 1, 2)
+    srv.submit(ok)
+    srv.drain()
+    assert float(ok.result[0]) == 2.0 and srv.used == 0
