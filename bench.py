@@ -419,18 +419,18 @@ def run_aux(env, w, updates, reps=10):
     return out
 
 
-def run_large(pkg, n=800, steps=20, seed=3):
-    """800^3 x 2 FP32 scaled-deviation form vs the FP64 path: an oxygen-like
-    substrate (D=1e5, decay 0.1, Dirichlet 38) and a signal (D=1e3, decay 0.4,
-    Dirichlet 0); 1e6 agents in a radius-6 mm spheroid at the centre, density
-    weighted toward the rim, uptake 10/min on substrate 0 and secretion 1/min
-    on substrate 1 (saturation density 40 on substrate 1)."""
+def large_workload(n=800, seed=3, na=1_000_000):
+    """Config-3 workload: 800^3 x 2, an oxygen-like substrate (D=1e5, decay
+    0.1, Dirichlet 38) and a signal (D=1e3, decay 0.4, Dirichlet 0) on all six
+    faces, the field starting at the Dirichlet values; 1e6 agents in a
+    radius-6 mm spheroid at the centre, density weighted toward the rim,
+    uptake 10/min on substrate 0 and secretion 1/min on substrate 1
+    (saturation density 40 on substrate 1), volume 2494 um^3."""
     rng = np.random.default_rng(seed)
     ns = 2
     D = np.array([1e5, 1e3])
     decay = np.array([0.1, 0.4])
     dval = np.array([38.0, 0.0])
-    na = 1_000_000
     # radius r in [0, R] with density ~ exp(-((R - r)/w)^2) r^2 (rim-weighted)
     R = 6000.0
     r = R * (1 - np.abs(rng.normal(0, 0.15, na)))
@@ -447,7 +447,20 @@ def run_large(pkg, n=800, steps=20, seed=3):
     S[:, 1] = 1.0
     rs = np.zeros((na, ns))
     rs[:, 1] = 40.0
-    c1, c2, e2 = agent_constants(0.01, 20.0 ** 3, np.full(na, 2494.0), S, U, np.zeros((na, ns)), rs)
+    V = np.full(na, 2494.0)
+    E = np.zeros((na, ns))
+    c1, c2, e2 = agent_constants(0.01, 20.0 ** 3, V, S, U, E, rs)
+    return dict(n=n, ns=ns, D=D, decay=decay, dval=dval, voxel=voxel, c1=c1, c2=c2, e2=e2,
+                S=S, U=U, E=E, rs=rs, V=V)
+
+
+def run_large(pkg, n=800, steps=20, seed=3):
+    """800^3 x 2 FP32 scaled-deviation form vs the FP64 path on the config-3
+    workload (large_workload)."""
+    w = large_workload(n, seed)
+    ns, D, decay, dval = w["ns"], w["D"], w["decay"], w["dval"]
+    voxel, c1, c2, e2 = w["voxel"], w["c1"], w["c2"], w["e2"]
+    na = voxel.size
     field = np.empty(ns * n ** 3)
     field[: n ** 3] = 38.0
     field[n ** 3:] = 0.0

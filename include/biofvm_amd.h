@@ -86,6 +86,20 @@ int bfa_set_coefficients(bfa_env *env, const double *D, const double *decay);
  * values of the last registered face containing it, else 1.0 (:632). */
 int bfa_set_dirichlet(bfa_env *env, const uint8_t *active, const uint8_t *face_registered,
                       const double *face_values);
+/* Per-voxel Dirichlet values of the boundary voxels: the face-voxel entries of
+ * Microenvironment::dirichlet_value_vectors (BioFVM/BioFVM_microenvironment.h,
+ * written by add_dirichlet_node / update_dirichlet_node, :218-245) that
+ * apply_dirichlet_boundaries_conditions reads per voxel (:348-446).  planes
+ * holds six planes back to back, each in reference voxel order with the
+ * substrate fastest:
+ *   xmin, xmax: [j][k][d]   (ny*nz*ns values each)
+ *   ymin, ymax: [i][k][d]   (nx*nz*ns values each)
+ *   zmin, zmax: [i][j][d]   (nx*ny*ns values each)
+ * A voxel on several faces appears in each of their planes and must carry the
+ * same values there (BFA_ERR_ARG otherwise).  The planes replace the face
+ * values of bfa_set_dirichlet until called again with planes == NULL; the
+ * activation flags stay those of bfa_set_dirichlet. */
+int bfa_set_dirichlet_planes(bfa_env *env, const double *planes);
 
 /* Host <-> device field transfer, FP64 on the host in the given layout.  For
  * BFA_FP32 an upload also picks the deviation basis per substrate (b = the
@@ -113,8 +127,43 @@ int bfa_get_fp32_basis(const bfa_env *env, double *b, double *s);
 int bfa_set_agents(bfa_env *env, int n, const int32_t *voxel, const uint8_t *active,
                    const double *c1, const double *c2, const double *e2);
 
+/* Which reference solver the step reproduces.  The two differ only in the
+ * Dirichlet passes; their recurrences are the same statements at world size 1.
+ *   BFA_LOD_3D_BLOCKING: Dirichlet before each sweep and after the z sweep
+ *     (diffusion_decay_solver__constant_coefficients_LOD_3D_BLOCKING,
+ *     BioFVM/BioFVM_solvers.cpp:345,682,731,778), which
+ *     initialize_microenvironment selects (BioFVM_microenvironment.cpp:2506).
+ *   BFA_LOD_3D_AVX256D: Dirichlet before each sweep only
+ *     (diffusion_decay_solver__constant_coefficients_LOD_3D_AVX256D,
+ *     BioFVM/BioFVM_solvers.cpp:1072,1510,1618), the Microenvironment
+ *     constructor's default (BioFVM_microenvironment.cpp:143).
+ * Default BFA_LOD_3D_BLOCKING. */
+enum bfa_solver { BFA_LOD_3D_BLOCKING = 0, BFA_LOD_3D_AVX256D = 1 };
+int bfa_set_solver(bfa_env *env, int variant);
+int bfa_get_solver(const bfa_env *env);
+
+/* Internalised-substrate tracking, what Basic_Agent::simulate_secretion_and_uptake
+ * does when default_microenvironment_options.
+ * track_internalized_substrates_in_each_agent is set
+ * (BioFVM/BioFVM_basic_agent.cpp:520-532,553-556): before an agent's update
+ *   internalized[a][d] -= W*((1 - c2)*rho + c1)/c2     (W = dx^3, rho the
+ *   density the agent sees), and after it internalized[a][d] -= e1[a][d].
+ * The totals live on the device, [n][ns] in the caller's agent order of the
+ * last bfa_set_agents / bfa_set_agents_state call (n agents), and are updated
+ * by every bfa_secretion_uptake and bfa_step.
+ * bfa_track_internalized(env, 1, initial, e1) switches tracking on: initial
+ * [n][ns] are the current totals (NULL: zeros); e1 [n][ns] is
+ * cell_source_sink_solver_temp_export1 (NULL: after bfa_set_agents_state its
+ * export_rate * dt, after bfa_set_agents the e1 given before or zeros).
+ * enable = 0 switches it off.  A later agent call with the same n keeps the
+ * totals (and, from bfa_set_agents, the e1); one with another n switches
+ * tracking off.  bfa_get_internalized copies the totals to out [n][ns]. */
+int bfa_track_internalized(bfa_env *env, int enable, const double *initial, const double *e1);
+int bfa_get_internalized(bfa_env *env, double *out);
+
 /* One simulate_diffusion_decay(dt): Dirichlet, x sweep, Dirichlet, y sweep,
- * Dirichlet, z sweep, Dirichlet (BioFVM/BioFVM_solvers.cpp:345-778). */
+ * Dirichlet, z sweep, Dirichlet (BioFVM/BioFVM_solvers.cpp:345-778); without
+ * the last pass under BFA_LOD_3D_AVX256D. */
 int bfa_diffusion_decay(bfa_env *env, double dt);
 /* One simulate_cell_sources_and_sinks pass with the constants of
  * bfa_set_agents, composed per voxel in ascending agent order. */

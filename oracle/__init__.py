@@ -54,6 +54,8 @@ def lib():
         L.orc_dirichlet_planes.argtypes = [ctypes.c_int] * 4 + [_up, _dp] + [_dp] * 6
         L.orc_apply_dirichlet.argtypes = [_dp] + [ctypes.c_int] * 4 + [_up] + [_dp] * 6
         L.orc_lod_step.argtypes = [_dp] + [ctypes.c_int] * 4 + [_dp] * 7 + [_up] + [_dp] * 6
+        L.orc_lod_step_variant.argtypes = ([_dp] + [ctypes.c_int] * 4 + [_dp] * 7 + [_up] + [_dp] * 6
+                                           + [ctypes.c_int])
         L.orc_agent_constants.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                           ctypes.c_double] + [_dp] * 9
         L.orc_secretion_uptake.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _ip, _up,
@@ -125,7 +127,7 @@ class Oracle:
     """
 
     def __init__(self, nx, ny, nz, ns, dx, D, decay, dirichlet_active=None,
-                 face_registered=None, face_values=None):
+                 face_registered=None, face_values=None, solver="blocking"):
         self.nx, self.ny, self.nz, self.ns = int(nx), int(ny), int(nz), int(ns)
         self.dx = float(dx)
         self.dV = self.dx ** 3
@@ -141,7 +143,19 @@ class Oracle:
         self.face_registered = np.ascontiguousarray(face_registered, dtype=np.uint8).reshape(6)
         self.face_values = _f64(face_values).reshape(6, ns)
         self.planes = dirichlet_planes(nx, ny, nz, ns, self.face_registered, self.face_values)
+        if solver not in ("blocking", "avx256d"):
+            raise ValueError(f"unknown solver {solver!r}")
+        # LOD_3D_AVX256D has no Dirichlet pass after the z sweep
+        self.final_dirichlet = 1 if solver == "blocking" else 0
         self.dt = None
+
+    def set_planes(self, planes):
+        """Per-voxel face values (dirichlet_value_vectors on the boundary):
+        six arrays xmin, xmax [ny][nz][ns], ymin, ymax [nx][nz][ns], zmin, zmax
+        [nx][ny][ns], as dirichlet_planes returns them."""
+        nx, ny, nz, ns = self.nx, self.ny, self.nz, self.ns
+        shapes = [(ny, nz, ns)] * 2 + [(nx, nz, ns)] * 2 + [(nx, ny, ns)] * 2
+        self.planes = [_f64(p).reshape(sh).copy() for p, sh in zip(planes, shapes)]
 
     def precompute(self, dt):
         if self.dt == dt:
@@ -166,10 +180,10 @@ class Oracle:
         self.check(u)
         self.precompute(dt)
         c1 = self.consts[0]
-        lib().orc_lod_step(_d(u), self.nx, self.ny, self.nz, self.ns, _d(c1),
-                           _d(self.tx[0]), _d(self.tx[1]), _d(self.ty[0]), _d(self.ty[1]),
-                           _d(self.tz[0]), _d(self.tz[1]), self.active.ctypes.data_as(_up),
-                           *[_d(p) for p in self.planes])
+        lib().orc_lod_step_variant(_d(u), self.nx, self.ny, self.nz, self.ns, _d(c1),
+                                   _d(self.tx[0]), _d(self.tx[1]), _d(self.ty[0]), _d(self.ty[1]),
+                                   _d(self.tz[0]), _d(self.tz[1]), self.active.ctypes.data_as(_up),
+                                   *[_d(p) for p in self.planes], self.final_dirichlet)
 
     def secretion_uptake(self, u, agents, internalized=None):
         """Serial simulate_secretion_and_uptake over ``agents`` (an AgentSet)."""

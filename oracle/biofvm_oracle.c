@@ -13,6 +13,8 @@
  *                               axpy at BioFVM/BioFVM_vector.cpp:331-338
  *   - LOD_3D_BLOCKING step,     BioFVM/BioFVM_solvers.cpp:102-780
  *     world size 1, granularity 1
+ *   - LOD_3D_AVX256D step at    BioFVM/BioFVM_solvers.cpp:796-1665
+ *     world size 1 (no Dirichlet pass after the z sweep)
  *   - Dirichlet faces           BioFVM/BioFVM_microenvironment.cpp:348-446
  *   - Dirichlet value table     BioFVM/BioFVM_microenvironment.cpp:218-226
  *     (add_dirichlet_node),     :632 (default 1.0), :2649-2786 (face order)
@@ -242,12 +244,35 @@ void orc_apply_dirichlet(double *u, int nx, int ny, int nz, int ns,
  * y sweep (:684-729), Dirichlet, z sweep (:733-777), Dirichlet.  The OpenMP
  * work split is the reference's: x lines over (j,k), y over collapse(i,k),
  * z over collapse(i,j). */
+void orc_lod_step_variant(double *u, int nx, int ny, int nz, int ns, const double *c1,
+                          const double *denx, const double *cpx, const double *deny,
+                          const double *cpy, const double *denz, const double *cpz,
+                          const unsigned char *active, const double *p0,
+                          const double *p1, const double *p2, const double *p3,
+                          const double *p4, const double *p5, int final_dirichlet);
+
 void orc_lod_step(double *u, int nx, int ny, int nz, int ns, const double *c1,
                   const double *denx, const double *cpx, const double *deny,
                   const double *cpy, const double *denz, const double *cpz,
                   const unsigned char *active, const double *p0,
                   const double *p1, const double *p2, const double *p3,
                   const double *p4, const double *p5)
+{
+    orc_lod_step_variant(u, nx, ny, nz, ns, c1, denx, cpx, deny, cpy, denz, cpz, active,
+                         p0, p1, p2, p3, p4, p5, 1);
+}
+
+/* The step with or without the Dirichlet pass after the z sweep.
+ * final_dirichlet = 1 is LOD_3D_BLOCKING (BioFVM/BioFVM_solvers.cpp:778);
+ * final_dirichlet = 0 is LOD_3D_AVX256D at world size 1, whose three passes
+ * come before the x, y and z sweeps only (:1072, :1510, :1618) and whose
+ * recurrences at rank 0 are the same statements. */
+void orc_lod_step_variant(double *u, int nx, int ny, int nz, int ns, const double *c1,
+                          const double *denx, const double *cpx, const double *deny,
+                          const double *cpy, const double *denz, const double *cpz,
+                          const unsigned char *active, const double *p0,
+                          const double *p1, const double *p2, const double *p3,
+                          const double *p4, const double *p5, int final_dirichlet)
 {
     const size_t ij = (size_t)ny * nz * ns; /* thomas_i_jump */
     const size_t jj = (size_t)nz * ns;      /* thomas_j_jump */
@@ -333,7 +358,8 @@ void orc_lod_step(double *u, int nx, int ny, int nz, int ns, const double *c1,
             }
         }
 
-    orc_apply_dirichlet(u, nx, ny, nz, ns, active, p0, p1, p2, p3, p4, p5);
+    if (final_dirichlet)
+        orc_apply_dirichlet(u, nx, ny, nz, ns, active, p0, p1, p2, p3, p4, p5);
 }
 
 /* Basic_Agent::set_internal_uptake_constants(dt)

@@ -67,6 +67,11 @@ def lib():
     L.bfa_last_error.restype = ctypes.c_char_p
     L.bfa_set_coefficients.argtypes = [_vp, _dp, _dp]
     L.bfa_set_dirichlet.argtypes = [_vp, _u8, _u8, _dp]
+    L.bfa_set_dirichlet_planes.argtypes = [_vp, _dp]
+    L.bfa_set_solver.argtypes = [_vp, ctypes.c_int]
+    L.bfa_track_internalized.argtypes = [_vp, ctypes.c_int, _dp, _dp]
+    L.bfa_get_internalized.argtypes = [_vp, _dp]
+    L.bfa_get_solver.argtypes = [_vp]
     L.bfa_upload.argtypes = [_vp, _dp, ctypes.c_int]
     L.bfa_download.argtypes = [_vp, _dp, ctypes.c_int]
     L.bfa_fill.argtypes = [_vp, _dp]
@@ -231,6 +236,35 @@ class Microenvironment:
                                             self.face_registered.ctypes.data_as(_u8),
                                             _d(self.face_values)), "set_dirichlet")
 
+    def set_dirichlet_planes(self, planes):
+        """Per-voxel Dirichlet values of the six face planes (the face-voxel
+        entries of dirichlet_value_vectors): a sequence of six arrays xmin, xmax
+        [ny][nz][ns], ymin, ymax [nx][nz][ns], zmin, zmax [nx][ny][ns], or one
+        array of them back to back; None returns to the face values."""
+        if planes is None:
+            self._check(lib().bfa_set_dirichlet_planes(self._h, None), "set_dirichlet_planes")
+            return
+        if not isinstance(planes, np.ndarray):
+            planes = np.concatenate([np.asarray(p, np.float64).ravel() for p in planes])
+        nx, ny, nz, ns = self.nx, self.ny, self.nz, self.ns
+        total = 2 * (ny * nz + nx * nz + nx * ny) * ns
+        flat = _f64(planes, total)
+        self._check(lib().bfa_set_dirichlet_planes(self._h, _d(flat)), "set_dirichlet_planes")
+
+    SOLVERS = {"blocking": 0, "avx256d": 1}
+
+    def set_solver(self, name):
+        """Which reference solver the step reproduces: "blocking"
+        (LOD_3D_BLOCKING, Dirichlet after the z sweep too) or "avx256d"
+        (LOD_3D_AVX256D, no Dirichlet pass after the z sweep)."""
+        if name not in self.SOLVERS:
+            raise BioFVMError(f"unknown solver {name!r}")
+        self._check(lib().bfa_set_solver(self._h, self.SOLVERS[name]), "set_solver")
+
+    def solver(self):
+        v = lib().bfa_get_solver(self._h)
+        return {0: "blocking", 1: "avx256d"}[v]
+
     def set_substrate_dirichlet_activation(self, index, flag):
         """Microenvironment::set_substrate_dirichlet_activation
         (BioFVM/BioFVM_microenvironment.cpp:265-273)."""
@@ -284,6 +318,7 @@ class Microenvironment:
         self._check(lib().bfa_set_agents_state(self._h, n, _d(pos), _d(box), _d(vol), *(_d(a) for a in args),
                                                act.ctypes.data_as(_u8) if act is not None else None,
                                                float(dt)), "set_agents_state")
+        self._n_agents = n
 
     def set_bulk_rates(self, S, T, U, layout="ref"):
         """Per-value supply rate S, supply target T, uptake rate U for
@@ -372,6 +407,29 @@ class Microenvironment:
         self._check(lib().bfa_set_agents(self._h, n, voxel.ctypes.data_as(_i32),
                                          None if act is None else act.ctypes.data_as(_u8),
                                          _d(c1), _d(c2), _d(e2)), "set_agents")
+        self._n_agents = n
+
+    def track_internalized(self, initial=None, e1=None):
+        """Switch on internalised-substrate tracking for the current agents
+        (track_internalized_substrates_in_each_agent): initial totals [n][ns]
+        (None: zeros) and temp_export1 e1 [n][ns] (None: from
+        set_agents_state's export rates, or zeros)."""
+        n = getattr(self, "_n_agents", 0)
+        cnt = n * self.ns
+        ini = None if initial is None else _f64(initial, cnt)
+        ee = None if e1 is None else _f64(e1, cnt)
+        self._check(lib().bfa_track_internalized(self._h, 1, None if ini is None else _d(ini),
+                                                 None if ee is None else _d(ee)), "track_internalized")
+
+    def stop_tracking_internalized(self):
+        self._check(lib().bfa_track_internalized(self._h, 0, None, None), "track_internalized")
+
+    def internalized(self):
+        """Per-agent internalised totals [n][ns] (caller's agent order)."""
+        n = getattr(self, "_n_agents", 0)
+        out = np.empty((n, self.ns))
+        self._check(lib().bfa_get_internalized(self._h, _d(out)), "get_internalized")
+        return out
 
     # -- stepping ------------------------------------------------------------------
     def simulate_diffusion_decay(self, dt):

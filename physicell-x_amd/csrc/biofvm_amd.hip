@@ -57,6 +57,11 @@ struct bfa_env {
     bool coef_set = false;
     std::vector<uint8_t> active, reg;
     std::vector<double> fvals;  // [6][ns]
+    // per-voxel Dirichlet face planes (empty: face table), FP64 in the device
+    // plane order [ns][shell] (see SweepArgs::plane), and their device copy in T
+    std::vector<double> hplanes;
+    long long shell = 0;
+    void* planeT = nullptr;
     std::vector<double> b, s;   // FP32 basis
     bool basis_fixed = false;
 
@@ -83,6 +88,14 @@ struct bfa_env {
     void* agB = nullptr;  // c2 (FP64)
     void* agC = nullptr;  // e2 (FP64)
     std::vector<double> h_c1, h_c2, h_e2;  // host copies of the sorted constants
+    // internalised-substrate tracking: the agent count of the last agent call
+    // (caller order), the caller index of each sorted agent, the net export
+    // e1 = E*dt [n_in][ns] (caller order; nullptr = none), and when tracking
+    // is on the per-agent totals [n_in][ns]
+    int n_in = 0;
+    int* ag_orig = nullptr;
+    double* ag_e1 = nullptr;
+    double* trk = nullptr;
 
     // graph of one step
     hipGraph_t graph = nullptr;
@@ -96,6 +109,7 @@ struct bfa_env {
     int L[3] = {0, 0, 0};
     int R = 0;
     int TX[3] = {0, 64, 64};
+    int variant = BFA_LOD_3D_BLOCKING;  // which reference solver's Dirichlet passes
     int slab = 0;  // k-planes per x/y slab pair (0: whole-volume sweeps)
     int split = 1;  // substrate groups swept on parallel streams (1 or 2)
 
@@ -291,10 +305,14 @@ bfa::SweepArgs<T> sweep_args(const bfa_env* e, int axis) {
     a.k = reinterpret_cast<const T*>(e->kT);
     a.beta = reinterpret_cast<const T*>(e->betaT);
     a.dval = reinterpret_cast<const T*>(e->dvalT);
+    a.plane = e->hplanes.empty() ? nullptr : reinterpret_cast<const T*>(e->planeT);
+    a.shell = e->shell;
     unsigned long long act = 0;
     for (int d = 0; d < e->ns; ++d)
         if (e->active[d]) act |= 1ull << d;
-    a.active = act;
+    // the z sweep's only Dirichlet pass is the one after it, which
+    // LOD_3D_AVX256D does not make
+    a.active = (axis == 2 && e->variant == BFA_LOD_3D_AVX256D) ? 0ull : act;
     unsigned reg = 0;
     for (int f = 0; f < 6; ++f)
         if (e->reg[f]) reg |= 1u << f;
@@ -509,18 +527,20 @@ int launch_agents(bfa_env* e) {
     if (e->nseg == 0) return BFA_OK;
     const long long work = (long long)e->nseg * e->ns;
     const unsigned blocks = (unsigned)((work + 255) / 256);
+    const double W = e->dx * e->dx * e->dx;  // voxel volume
     if (e->prec == BFA_FP32) {
         hipLaunchKernelGGL(bfa::k_agents_f32, dim3(blocks), dim3(256), 0, e->stream,
                            reinterpret_cast<float*>(e->u), e->N, e->ns, e->nseg, e->nag,
                            e->seg_off, e->seg_vox, reinterpret_cast<const double*>(e->agA),
                            reinterpret_cast<const double*>(e->agB),
-                           reinterpret_cast<const double*>(e->agC), e->params);
+                           reinterpret_cast<const double*>(e->agC), e->params, e->trk, e->ag_orig,
+                           e->ag_e1, W);
     } else {
         hipLaunchKernelGGL(bfa::k_agents_f64, dim3(blocks), dim3(256), 0, e->stream,
                            reinterpret_cast<double*>(e->u), e->N, e->ns, e->nseg, e->nag,
                            e->seg_off, e->seg_vox, reinterpret_cast<const double*>(e->agA),
                            reinterpret_cast<const double*>(e->agB),
-                           reinterpret_cast<const double*>(e->agC));
+                           reinterpret_cast<const double*>(e->agC), e->trk, e->ag_orig, e->ag_e1, W);
     }
     LAUNCH_CHECK(e, "agents");
     return BFA_OK;
@@ -561,8 +581,34 @@ int upload_tables(bfa_env* e) {
         }
     HIP_TRY(e, hipMemcpyAsync(e->dvalT, dv.data(), dv.size() * sizeof(T), hipMemcpyHostToDevice,
                               e->stream));
+    if (!e->hplanes.empty()) {
+        std::vector<T> pv(e->hplanes.size());
+        for (int d = 0; d < ns; ++d)
+            for (long long q = 0; q < e->shell; ++q) {
+                const size_t o = (size_t)d * e->shell + q;
+                pv[o] = (T)((e->hplanes[o] - e->b[d]) / e->s[d]);
+            }
+        HIP_TRY(e, hipMemcpyAsync(e->planeT, pv.data(), pv.size() * sizeof(T), hipMemcpyHostToDevice,
+                                  e->stream));
+        HIP_TRY(e, hipStreamSynchronize(e->stream));  // pv is released on return
+    }
     HIP_TRY(e, hipStreamSynchronize(e->stream));
     return BFA_OK;
+}
+
+// FP32 basis value b of an active substrate: the Dirichlet value it is held
+// at (the mean plane value with per-voxel planes; else the value of the last
+// registered face, or the default 1.0), so face voxels store v near 0.
+double dirichlet_basis(const bfa_env* e, int d) {
+    if (!e->hplanes.empty()) {
+        double acc = 0.0;
+        const double* pl = e->hplanes.data() + (size_t)d * e->shell;
+        for (long long q = 0; q < e->shell; ++q) acc += pl[q];
+        return acc / (double)e->shell;
+    }
+    for (int f = 5; f >= 0; --f)
+        if (e->reg[f]) return e->fvals[(size_t)f * e->ns + d];
+    return 1.0;
 }
 
 template <typename T>
@@ -664,6 +710,12 @@ int reduce(bfa_env* e, const void* x, bool f32, long long count, long long strid
                               e->stream));
     HIP_TRY(e, hipStreamSynchronize(e->stream));
     return BFA_OK;
+}
+
+// A new agent list of another length has other agent identities: the
+// tracked totals (indexed by agent) no longer apply and tracking stops.
+void drop_tracking_if_resized(bfa_env* e, int n) {
+    if (e->trk && n != e->n_in) dfree(e->trk);
 }
 
 }  // namespace
@@ -790,6 +842,7 @@ void bfa_destroy(bfa_env* e) {
     dfree(e->kT);
     dfree(e->betaT);
     dfree(e->dvalT);
+    dfree(e->planeT);
     dfree(e->params);
     dfree(e->red_partial);
     dfree(e->red_out);
@@ -798,6 +851,9 @@ void bfa_destroy(bfa_env* e) {
     dfree(e->agA);
     dfree(e->agB);
     dfree(e->agC);
+    dfree(e->ag_orig);
+    dfree(e->ag_e1);
+    dfree(e->trk);
     dfree(e->bulk_st);
     dfree(e->bulk_ups);
     for (int i = 0; i < 2; ++i) {
@@ -834,6 +890,79 @@ int bfa_set_dirichlet(bfa_env* e, const uint8_t* active, const uint8_t* face_reg
     e->fvals.assign(face_values, face_values + 6 * (size_t)e->ns);
     e->pre_valid = false;
     invalidate_graph(e);
+    return BFA_OK;
+}
+
+int bfa_set_dirichlet_planes(bfa_env* e, const double* planes) {
+    if (!e) return BFA_ERR_ARG;
+    const int nx = e->nx, ny = e->ny, nz = e->nz, ns = e->ns;
+    if (!planes) {
+        invalidate_graph(e);
+        e->hplanes.clear();
+        e->hplanes.shrink_to_fit();
+        dfree(e->planeT);
+        e->pre_valid = false;
+        return BFA_OK;
+    }
+    const long long nyz = (long long)ny * nz, nxz = (long long)nx * nz, nxy = (long long)nx * ny;
+    const long long shell = 2 * (nyz + nxz + nxy);
+    // input: reference voxel order per plane, substrate fastest:
+    //   xmin/xmax [j][k][d], ymin/ymax [i][k][d], zmin/zmax [i][j][d]
+    // device: [d][shell] with xmin/xmax [k][j], ymin/ymax [k][i], zmin/zmax [j][i]
+    const long long in_off[6] = {0, nyz, 2 * nyz, 2 * nyz + nxz, 2 * (nyz + nxz), 2 * (nyz + nxz) + nxy};
+    std::vector<double> h((size_t)shell * ns);
+    for (int f = 0; f < 6; ++f) {
+        const int A = f < 2 ? ny : nx;        // slow index of the input plane
+        const int B = f < 4 ? nz : ny;        // fast index
+        for (int a = 0; a < A; ++a)
+            for (int bq = 0; bq < B; ++bq) {
+                const long long qin = in_off[f] + (long long)a * B + bq;
+                // device index: x faces k*ny + j (a = j, bq = k); y faces
+                // k*nx + i (a = i, bq = k); z faces j*nx + i (a = i, bq = j)
+                const long long qdev = in_off[f] + (f < 4 ? (long long)bq * A + a : (long long)bq * nx + a);
+                for (int d = 0; d < ns; ++d) {
+                    const double v = planes[(size_t)qin * ns + d];
+                    if (!std::isfinite(v))
+                        return fail(e, BFA_ERR_ARG, "Dirichlet plane %d entry %lld substrate %d is not finite", f,
+                                    qin, d);
+                    h[(size_t)d * shell + qdev] = v;
+                }
+            }
+    }
+    // a voxel on several faces has one value (dirichlet_value_vectors[voxel])
+    auto at = [&](int i, int j, int k, int d, int f) {
+        long long q = 0;
+        if (f < 2) q = in_off[f] + (long long)k * ny + j;
+        else if (f < 4) q = in_off[f] + (long long)k * nx + i;
+        else q = in_off[f] + (long long)j * nx + i;
+        return h[(size_t)d * shell + q];
+    };
+    for (int d = 0; d < ns; ++d)
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; i += (j == 0 || j == ny - 1 || k == 0 || k == nz - 1) ? 1 : std::max(1, nx - 1)) {
+                    int faces[6], nf = 0;
+                    if (i == 0) faces[nf++] = 0;
+                    if (i == nx - 1) faces[nf++] = 1;
+                    if (j == 0) faces[nf++] = 2;
+                    if (j == ny - 1) faces[nf++] = 3;
+                    if (k == 0) faces[nf++] = 4;
+                    if (k == nz - 1) faces[nf++] = 5;
+                    for (int q = 1; q < nf; ++q)
+                        if (at(i, j, k, d, faces[q]) != at(i, j, k, d, faces[0]))
+                            return fail(e, BFA_ERR_ARG,
+                                        "voxel (%d,%d,%d) substrate %d has different values in Dirichlet planes %d "
+                                        "and %d (one value per voxel)", i, j, k, d, faces[0], faces[q]);
+                }
+    invalidate_graph(e);
+    if (!e->planeT || e->shell != shell) {
+        dfree(e->planeT);
+        int rc = dalloc(e, &e->planeT, (size_t)shell * ns * e->tsize);
+        if (rc) return rc;
+    }
+    e->shell = shell;
+    e->hplanes.swap(h);
+    e->pre_valid = false;
     return BFA_OK;
 }
 
@@ -883,18 +1012,7 @@ int bfa_upload(bfa_env* e, const double* field, int layout) {
         std::vector<double> sum(e->ns), mx(e->ns);
         if ((rc = reduce(e, stage, false, e->N, stride, dstride, 0, nullptr, sum.data())))
             return cleanup(rc);
-        for (int d = 0; d < e->ns; ++d) {
-            double b = sum[d] / (double)e->N;
-            if (e->active[d]) {
-                b = 1.0;
-                for (int f = 5; f >= 0; --f)
-                    if (e->reg[f]) {
-                        b = e->fvals[(size_t)f * e->ns + d];
-                        break;
-                    }
-            }
-            e->b[d] = b;
-        }
+        for (int d = 0; d < e->ns; ++d) e->b[d] = e->active[d] ? dirichlet_basis(e, d) : sum[d] / (double)e->N;
         std::vector<double> bdev(e->b);
         double* bref = nullptr;
         if ((rc = dalloc(e, &bref, e->ns * 8))) return cleanup(rc);
@@ -957,15 +1075,7 @@ int bfa_fill(bfa_env* e, const double* values) {
         if (!std::isfinite(values[d])) return fail(e, BFA_ERR_ARG, "value of substrate %d is not finite", d);
     if (e->prec == BFA_FP32 && !e->basis_fixed) {
         for (int d = 0; d < ns; ++d) {
-            double b = values[d];
-            if (e->active[d]) {
-                b = 1.0;
-                for (int f = 5; f >= 0; --f)
-                    if (e->reg[f]) {
-                        b = e->fvals[(size_t)f * ns + d];
-                        break;
-                    }
-            }
+            const double b = e->active[d] ? dirichlet_basis(e, d) : values[d];
             e->b[d] = b;
             const double dev = std::fabs(values[d] - b);
             int ex = 0;
@@ -1075,6 +1185,12 @@ int bfa_set_agents(bfa_env* e, int n, const int32_t* voxel, const uint8_t* activ
     dfree(e->agA);
     dfree(e->agB);
     dfree(e->agC);
+    dfree(e->ag_orig);
+    drop_tracking_if_resized(e, n);
+    // host constants carry no e1: tracking kept on for the same agents keeps
+    // the e1 it had, else bfa_track_internalized sets it
+    if (!e->trk) dfree(e->ag_e1);
+    e->n_in = n;
     e->nag = e->nseg = 0;
     e->h_c1.clear();
     e->h_c2.clear();
@@ -1135,8 +1251,49 @@ int bfa_set_agents(bfa_env* e, int n, const int32_t* voxel, const uint8_t* activ
     HIP_TRY(e, hipMemcpy(e->agA, e->h_c1.data(), cnt * 8, hipMemcpyHostToDevice));
     HIP_TRY(e, hipMemcpy(e->agB, e->h_c2.data(), cnt * 8, hipMemcpyHostToDevice));
     HIP_TRY(e, hipMemcpy(e->agC, e->h_e2.data(), cnt * 8, hipMemcpyHostToDevice));
+    std::vector<int> orig(m);
+    for (int r = 0; r < m; ++r) orig[r] = idx[order[r]];
+    if ((rc = dalloc(e, &e->ag_orig, (size_t)m * sizeof(int)))) return rc;
+    HIP_TRY(e, hipMemcpy(e->ag_orig, orig.data(), (size_t)m * sizeof(int), hipMemcpyHostToDevice));
     e->nag = m;
     e->nseg = nseg;
+    return BFA_OK;
+}
+
+int bfa_track_internalized(bfa_env* e, int enable, const double* initial, const double* e1) {
+    if (!e) return BFA_ERR_ARG;
+    invalidate_graph(e);
+    if (!enable) {
+        dfree(e->trk);
+        return BFA_OK;
+    }
+    const int n = e->n_in;
+    if (n <= 0) return fail(e, BFA_ERR_STATE, "no agents: call bfa_set_agents or bfa_set_agents_state first");
+    const size_t cnt = (size_t)n * e->ns;
+    if (initial)
+        for (size_t q = 0; q < cnt; ++q)
+            if (!std::isfinite(initial[q])) return fail(e, BFA_ERR_ARG, "internalized value %zu is not finite", q);
+    int rc;
+    if (!e->trk && (rc = dalloc(e, &e->trk, cnt * 8))) return rc;
+    if (e1 || !e->ag_e1) {
+        if (!e->ag_e1 && (rc = dalloc(e, &e->ag_e1, cnt * 8))) return rc;
+        if (e1)
+            HIP_TRY(e, hipMemcpy(e->ag_e1, e1, cnt * 8, hipMemcpyHostToDevice));
+        else
+            HIP_TRY(e, hipMemset(e->ag_e1, 0, cnt * 8));
+    }
+    if (initial)
+        HIP_TRY(e, hipMemcpy(e->trk, initial, cnt * 8, hipMemcpyHostToDevice));
+    else
+        HIP_TRY(e, hipMemset(e->trk, 0, cnt * 8));
+    return BFA_OK;
+}
+
+int bfa_get_internalized(bfa_env* e, double* out) {
+    if (!e || !out) return fail(e, BFA_ERR_ARG, "NULL argument");
+    if (!e->trk) return fail(e, BFA_ERR_STATE, "internalised-substrate tracking is off");
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    HIP_TRY(e, hipMemcpy(out, e->trk, (size_t)e->n_in * e->ns * 8, hipMemcpyDeviceToHost));
     return BFA_OK;
 }
 
@@ -1434,6 +1591,17 @@ int bfa_time_kernels(bfa_env* e, double dt, int reps, double* ms) {
     for (int k = 0; k < 5; ++k) ms[k] = acc[k] / reps;
     return BFA_OK;
 }
+
+int bfa_set_solver(bfa_env* e, int variant) {
+    if (!e) return BFA_ERR_ARG;
+    if (variant != BFA_LOD_3D_BLOCKING && variant != BFA_LOD_3D_AVX256D)
+        return fail(e, BFA_ERR_ARG, "unknown solver variant %d", variant);
+    if (variant != e->variant) invalidate_graph(e);
+    e->variant = variant;
+    return BFA_OK;
+}
+
+int bfa_get_solver(const bfa_env* e) { return e ? e->variant : -1; }
 
 int bfa_set_slab(bfa_env* e, int planes) {
     if (!e) return BFA_ERR_ARG;

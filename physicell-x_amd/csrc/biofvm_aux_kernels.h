@@ -71,10 +71,39 @@ __global__ void k_thomas(int n, int ns, const double* __restrict__ params, doubl
 // amplified near the map's fixed point).
 // Constants are stored [ns][nagents_sorted] so lanes of one substrate read
 // consecutive entries.
+// Internalised-substrate tracking of agent a (sorted position) before its
+// update of density rho (track_internalized_substrates_in_each_agent,
+// BioFVM/BioFVM_basic_agent.cpp:520-532): I -= W*((1 - c2)*rho + c1)/c2,
+// each operation rounded on its own as the reference's vector statements.
+// trk and e1 are [n][ns] in the caller's agent order; orig maps the sorted
+// position to that order.
+__device__ __forceinline__ void track_before(double* __restrict__ trk, const int* __restrict__ orig,
+                                             int ns, int d, int a, double rho, double c1, double c2,
+                                             double W) {
+#pragma clang fp contract(off)
+    double t = 1.0;
+    t -= c2;
+    t *= rho;
+    t += c1;
+    t /= c2;
+    t *= W;
+    double* I = trk + (size_t)orig[a] * ns + d;
+    *I = *I - t;
+}
+
+// ... and after it: I -= e1 (the net export, :553-556).
+__device__ __forceinline__ void track_after(double* __restrict__ trk, const int* __restrict__ orig,
+                                            const double* __restrict__ e1, int ns, int d, int a) {
+    const size_t o = (size_t)orig[a] * ns + d;
+    trk[o] = trk[o] - e1[o];
+}
+
 __global__ void k_agents_f64(double* __restrict__ u, long long N, int ns, int nseg,
                              int nag, const int* __restrict__ off,
                              const long long* __restrict__ vox, const double* __restrict__ c1,
-                             const double* __restrict__ c2, const double* __restrict__ e2) {
+                             const double* __restrict__ c2, const double* __restrict__ e2,
+                             double* __restrict__ trk, const int* __restrict__ orig,
+                             const double* __restrict__ e1, double W) {
     const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= (long long)nseg * ns) return;
     const int d = (int)(gid / nseg);
@@ -84,9 +113,11 @@ __global__ void k_agents_f64(double* __restrict__ u, long long N, int ns, int ns
     const int a0 = off[q], a1 = off[q + 1];
     const size_t o = (size_t)d * nag;
     for (int a = a0; a < a1; ++a) {
+        if (trk) track_before(trk, orig, ns, d, a, rho, c1[o + a], c2[o + a], W);
         rho += c1[o + a];
         rho /= c2[o + a];
         rho += e2[o + a];
+        if (trk) track_after(trk, orig, e1, ns, d, a);
     }
     *p = rho;
 }
@@ -94,7 +125,9 @@ __global__ void k_agents_f64(double* __restrict__ u, long long N, int ns, int ns
 __global__ void k_agents_f32(float* __restrict__ u, long long N, int ns, int nseg, int nag,
                              const int* __restrict__ off, const long long* __restrict__ vox,
                              const double* __restrict__ c1, const double* __restrict__ c2,
-                             const double* __restrict__ e2, const double* __restrict__ params) {
+                             const double* __restrict__ e2, const double* __restrict__ params,
+                             double* __restrict__ trk, const int* __restrict__ orig,
+                             const double* __restrict__ e1, double W) {
     const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= (long long)nseg * ns) return;
     const int d = (int)(gid / nseg);
@@ -105,9 +138,11 @@ __global__ void k_agents_f32(float* __restrict__ u, long long N, int ns, int nse
     const int a0 = off[q], a1 = off[q + 1];
     const size_t o = (size_t)d * nag;
     for (int a = a0; a < a1; ++a) {
+        if (trk) track_before(trk, orig, ns, d, a, rho, c1[o + a], c2[o + a], W);
         rho += c1[o + a];
         rho /= c2[o + a];
         rho += e2[o + a];
+        if (trk) track_after(trk, orig, e1, ns, d, a);
     }
     *p = (float)((rho - b) * rs);
 }
@@ -138,10 +173,9 @@ __global__ void k_dirichlet_faces(SweepArgs<T> a) {
     } else {
         return;
     }
-    const int slot = dir_slot(i, j, k, a.nx, a.ny, a.nz, a.reg);
     const long long v = ((long long)k * a.ny + j) * a.nx + i;
     for (int d = 0; d < a.ns; ++d)
-        if ((a.active >> d) & 1ull) a.u[(long long)d * a.N + v] = a.dval[slot * a.ns + d];
+        if ((a.active >> d) & 1ull) a.u[(long long)d * a.N + v] = dir_value(a, d, i, j, k);
 }
 
 // ---- reductions --------------------------------------------------------------

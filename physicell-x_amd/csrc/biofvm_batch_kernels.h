@@ -63,6 +63,8 @@ __device__ __forceinline__ SweepArgs<T> args_of(const EnvDesc<T>& e, int axis) {
     a.k = e.k;
     a.beta = e.beta;
     a.dval = e.dval;
+    a.plane = nullptr;  // batch members keep the face table
+    a.shell = 0;
     a.active = e.active;
     a.N = e.N;
     a.nx = e.nx;

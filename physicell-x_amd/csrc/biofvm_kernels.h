@@ -56,6 +56,14 @@ struct SweepArgs {
     const T* k;       // [ns] D*dt/dx^2
     const T* beta;    // [ns] deviation shift (FP32 form), unused for FP64
     const T* dval;    // [7][ns] Dirichlet value per face slot
+    // Per-voxel Dirichlet values of the six face planes, or nullptr to use
+    // dval.  Substrate d's planes start at plane + d*shell, in the order
+    //   xmin [k][j], xmax [k][j]   (ny*nz each)
+    //   ymin [k][i], ymax [k][i]   (nx*nz each)
+    //   zmin [j][i], zmax [j][i]   (nx*ny each)
+    // A voxel on several faces holds the same value in each of their planes.
+    const T* plane;
+    long long shell;  // 2*(ny*nz + nx*nz + nx*ny)
     unsigned long long active;  // bit d set: substrate d has Dirichlet on
     long long N;      // nx*ny*nz
     int nx, ny, nz, ns;
@@ -67,6 +75,36 @@ struct SweepArgs {
     int d0;
     long long first, count;
 };
+
+// Offsets of the six face planes inside one substrate's shell.
+struct PlaneOffsets {
+    long long xlo, xhi, ylo, yhi, zlo, zhi;
+};
+
+template <typename T>
+__device__ __forceinline__ PlaneOffsets plane_offsets(const SweepArgs<T>& a) {
+    const long long nyz = (long long)a.ny * a.nz, nxz = (long long)a.nx * a.nz,
+                    nxy = (long long)a.nx * a.ny;
+    return PlaneOffsets{0, nyz, 2 * nyz, 2 * nyz + nxz, 2 * (nyz + nxz), 2 * (nyz + nxz) + nxy};
+}
+
+// Dirichlet value of boundary voxel (i, j, k) of substrate d: the per-voxel
+// plane value when planes are set (any plane holding the voxel: they agree),
+// else the face table entry chosen by dir_slot.
+template <typename T>
+__device__ __forceinline__ T dir_value(const SweepArgs<T>& a, int d, int i, int j, int k) {
+    if (a.plane) {
+        const T* pl = a.plane + (long long)d * a.shell;
+        const PlaneOffsets o = plane_offsets(a);
+        if (i == 0) return pl[o.xlo + (long long)k * a.ny + j];
+        if (i == a.nx - 1) return pl[o.xhi + (long long)k * a.ny + j];
+        if (j == 0) return pl[o.ylo + (long long)k * a.nx + i];
+        if (j == a.ny - 1) return pl[o.yhi + (long long)k * a.nx + i];
+        if (k == 0) return pl[o.zlo + (long long)j * a.nx + i];
+        return pl[o.zhi + (long long)j * a.nx + i];
+    }
+    return a.dval[dir_slot(i, j, k, a.nx, a.ny, a.nz, a.reg) * a.ns + d];
+}
 
 // ---- chunk recurrences (positions s..s+L-1 of a padded line) ----------------
 // Coefficient arrays are padded to P*L entries with 1/den = c' = 0 past the
@@ -222,7 +260,7 @@ __global__ __launch_bounds__(1024) void k_sweep_x(SweepArgs<T> a, int R, int log
         for (int m = 0; m < L; ++m) {
             const int pos = s + m;
             if (pos < nx && (rowb || pos == 0 || pos == nx - 1))
-                x[m] = a.dval[dir_slot(pos, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
+                x[m] = dir_value(a, d, pos, jj, kk);
         }
     }
     if (DEV) {
@@ -236,7 +274,7 @@ __global__ __launch_bounds__(1024) void k_sweep_x(SweepArgs<T> a, int R, int log
         for (int m = 0; m < L; ++m) {
             const int pos = s + m;
             if (pos < nx && (rowb || pos == 0 || pos == nx - 1))
-                x[m] = a.dval[dir_slot(pos, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
+                x[m] = dir_value(a, d, pos, jj, kk);
         }
     }
     if (act) {
@@ -369,6 +407,7 @@ struct XCoef {
     T k, bt;          // D*dt/dx^2 and the deviation shift
     T v_lo, v_hi;     // Dirichlet values at the x ends of rows on no y/z face
     bool dact;        // Dirichlet on for this substrate
+    bool plane;       // per-voxel Dirichlet planes (values fetched per row)
 };
 
 template <typename T, int L, bool DEV>
@@ -383,6 +422,7 @@ __device__ __forceinline__ void x_coef_setup(const SweepArgs<T>& a, int d, int p
     c.k = a.k[d];
     c.bt = DEV ? a.beta[d] : T(0);
     c.dact = (a.active >> d) & 1ull;
+    c.plane = a.plane != nullptr;
     c.v_lo = a.dval[dir_slot_xonly(0, nx, a.reg) * a.ns + d];
     c.v_hi = a.dval[dir_slot_xonly(nx - 1, nx, a.reg) * a.ns + d];
     T za = T(1);
@@ -417,8 +457,9 @@ __device__ __forceinline__ void x_coef_setup(const SweepArgs<T>& a, int d, int p
 template <typename T, int L>
 struct XRow {
     T v[L];
-    T f_lo, f_hi, f_in;  // Dirichlet values of a row on a y or z face
+    T f_lo, f_hi, f_in;  // Dirichlet values: the row's x ends, and the interior of a row on a y or z face
     bool rowb;           // the row is on a y or z face and Dirichlet is on
+    const T* fp;         // per-voxel planes: the y or z plane row holding this face row's values
 };
 
 // The per-row Dirichlet part of x_row_fetch: rowb and the face values of
@@ -429,11 +470,24 @@ __device__ __forceinline__ void x_row_dir_fetch(const SweepArgs<T>& a, int d, in
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     q.f_lo = q.f_hi = q.f_in = T(0);
     q.rowb = false;
+    q.fp = nullptr;
     if (have && c.dact) {
         const int kk = r / ny;
         const int jj = r - kk * ny;
         q.rowb = (jj == 0 || jj == ny - 1 || kk == 0 || kk == nz - 1);
-        if (q.rowb) {
+        if (a.plane) {
+            // per-voxel values: the row's two x-end values (x planes are
+            // indexed by r = k*ny + j), and for a face row its y or z plane row
+            const T* pl = a.plane + (long long)d * a.shell;
+            const PlaneOffsets o = plane_offsets(a);
+            q.f_lo = pl[o.xlo + r];
+            q.f_hi = pl[o.xhi + r];
+            if (q.rowb)
+                q.fp = pl + (jj == 0        ? o.ylo + (long long)kk * nx
+                             : jj == ny - 1 ? o.yhi + (long long)kk * nx
+                             : kk == 0      ? o.zlo + (long long)jj * nx
+                                            : o.zhi + (long long)jj * nx);
+        } else if (q.rowb) {
             q.f_lo = a.dval[dir_slot(0, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
             q.f_hi = a.dval[dir_slot(nx - 1, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
             q.f_in = a.dval[dir_slot(-1, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
@@ -458,7 +512,16 @@ __device__ __forceinline__ void x_row_fetch(const SweepArgs<T>& a, int d, int r,
 template <typename T, int L>
 __device__ __forceinline__ void x_row_dirichlet(T (&cur)[L], int nx, int s, bool have,
                                                 const XCoef<T, L>& c, const XRow<T, L>& q) {
-    if (q.rowb) {
+    if (q.fp) {  // face row with per-voxel values: every node from its plane
+#pragma unroll
+        for (int m = 0; m < L; ++m)
+            if (s + m < nx) cur[m] = q.fp[s + m];
+    } else if (c.plane && have && c.dact) {  // per-voxel values at the row's x ends
+        if (s == 0) cur[0] = q.f_lo;
+#pragma unroll
+        for (int m = 0; m < L; ++m)
+            if (s + m == nx - 1) cur[m] = q.f_hi;
+    } else if (q.rowb) {
         dirichlet_row<T, L>(cur, s, nx, true, q.f_lo, q.f_hi, q.f_in);
     } else if (have && c.dact) {
         if (s == 0) cur[0] = c.v_lo;
@@ -661,6 +724,48 @@ __device__ __forceinline__ void col_dirichlet(T (&x)[L], const SweepArgs<T>& a, 
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const bool lineb = AXIS == 1 ? (ci == 0 || ci == nx - 1 || co == 0 || co == nz - 1)
                                  : (ci == 0 || ci == nx - 1 || co == 0 || co == ny - 1);
+    if (a.plane) {
+        // per-voxel values.  Line ends: the y (z) planes at (k, i) ((j, i)).
+        // A line on a face parallel to it reads every node from that face's
+        // plane: x faces are indexed k*ny + j, y faces k*nx + i, z faces
+        // j*nx + i, so along the line the index steps by `ls`.
+        const T* pl = a.plane + (long long)d * a.shell;
+        const PlaneOffsets o = plane_offsets(a);
+        if (lineb) {
+            long long b0, ls;
+            if (AXIS == 1) {  // column (i = ci, k = co), nodes j
+                if (ci == 0 || ci == nx - 1) {
+                    b0 = (ci == 0 ? o.xlo : o.xhi) + (long long)co * ny;
+                    ls = 1;
+                } else {
+                    b0 = (co == 0 ? o.zlo : o.zhi) + ci;
+                    ls = nx;
+                }
+            } else {  // column (i = ci, j = co), nodes k
+                if (ci == 0 || ci == nx - 1) {
+                    b0 = (ci == 0 ? o.xlo : o.xhi) + co;
+                    ls = ny;
+                } else {
+                    b0 = (co == 0 ? o.ylo : o.yhi) + ci;
+                    ls = nx;
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < L; ++m)
+                if (s + m < n) x[m] = pl[b0 + (long long)(s + m) * ls];
+        } else {
+            const long long e0 = AXIS == 1 ? o.ylo + (long long)co * nx + ci : o.zlo + (long long)co * nx + ci;
+            const long long e1 = AXIS == 1 ? o.yhi + (long long)co * nx + ci : o.zhi + (long long)co * nx + ci;
+            if (p == 0) x[0] = pl[e0];
+            if (p == P - 1) {
+                const T v = pl[e1];
+#pragma unroll
+                for (int m = 0; m < L; ++m)
+                    if (s + m == n - 1) x[m] = v;
+            }
+        }
+        return;
+    }
     const int s_lo = AXIS == 1 ? dir_slot(ci, 0, co, nx, ny, nz, a.reg)
                                : dir_slot(ci, co, 0, nx, ny, nz, a.reg);
     const int s_hi = AXIS == 1 ? dir_slot(ci, n - 1, co, nx, ny, nz, a.reg)

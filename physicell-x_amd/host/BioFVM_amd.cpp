@@ -102,9 +102,27 @@ void Basic_Agent::simulate_secretion_and_uptake(Microenvironment* m, double dt) 
     if (volume_is_changed) set_internal_uptake_constants(dt);
     double* rho = m->density_vector(current_voxel_index);
     const int ns = m->number_of_densities();
+    const bool track = m->track_internalized_substrates_in_each_agent;
+    if (track) {
+        // BioFVM/BioFVM_basic_agent.cpp:520-532: what the update below takes
+        // out of the voxel, W*((1 - c2)*rho + c1)/c2, goes into the agent
+        internalized_substrates.resize(ns, 0.0);
+        const double W = m->mesh.dV;
+        for (int d = 0; d < ns; ++d) {
+            double t = 1.0;
+            t -= cell_source_sink_solver_temp2[d];
+            t *= rho[d];
+            t += cell_source_sink_solver_temp1[d];
+            t /= cell_source_sink_solver_temp2[d];
+            t *= W;
+            internalized_substrates[d] -= t;
+        }
+    }
     for (int d = 0; d < ns; ++d) rho[d] += cell_source_sink_solver_temp1[d];
     for (int d = 0; d < ns; ++d) rho[d] /= cell_source_sink_solver_temp2[d];
     for (int d = 0; d < ns; ++d) rho[d] += cell_source_sink_solver_temp_export2[d];
+    if (track)
+        for (int d = 0; d < ns; ++d) internalized_substrates[d] -= cell_source_sink_solver_temp_export1[d];
 }
 
 // ---- microenvironment -------------------------------------------------------------------
@@ -134,6 +152,7 @@ void Microenvironment::set_density(int index, const std::string& nm, const std::
         decay_rates.resize(index + 1, 0.0);
         dirichlet_activation_vector.resize(index + 1, false);
         face_values.assign(6 * density_names.size(), 1.0);
+        dirichlet_planes.clear();
         host_field.assign((size_t)number_of_voxels() * density_names.size(), 0.0);
         if (env) {
             bfa_destroy(env);
@@ -158,6 +177,8 @@ void Microenvironment::resize_space_uniform(double x_start, double x_end, double
                                             double dx_new) {
     mesh.resize(x_start, x_end, y_start, y_end, z_start, z_end, dx_new, dx_new, dx_new);
     host_field.assign((size_t)number_of_voxels() * number_of_densities(), 0.0);
+    dirichlet_planes.clear();
+    dirichlet_flags.clear();
     if (env) {
         bfa_destroy(env);
         env = nullptr;
@@ -180,6 +201,7 @@ int Microenvironment::ensure_device() {
     host_newer = true;
     state_dirty = true;
     agents_sig.clear();
+    tracking_on_device = false;
     return BFA_OK;
 }
 
@@ -193,7 +215,8 @@ int Microenvironment::push_state() {
         std::vector<uint8_t> act(ns);
         for (int d = 0; d < ns; ++d) act[d] = dirichlet_activation_vector[d] ? 1 : 0;
         if ((rc = bfa_set_coefficients(env, diffusion_coefficients.data(), decay_rates.data())) ||
-            (rc = bfa_set_dirichlet(env, act.data(), face_registered, face_values.data()))) {
+            (rc = bfa_set_dirichlet(env, act.data(), face_registered, face_values.data())) ||
+            (rc = bfa_set_dirichlet_planes(env, dirichlet_planes.empty() ? nullptr : dirichlet_planes.data()))) {
             last_error = bfa_last_error(env);
             return rc;
         }
@@ -241,7 +264,119 @@ void Microenvironment::add_dirichlet_face(int face, const std::vector<double>& v
     }
     face_registered[face] = 1;
     std::copy(value.begin(), value.end(), face_values.begin() + (size_t)face * number_of_densities());
+    const int nx = mesh.x_nodes, ny = mesh.y_nodes, nz = mesh.z_nodes;
+    const bool track_nodes = !dirichlet_flags.empty() || !dirichlet_planes.empty();
+    if (track_nodes) {
+        // values written per voxel already: this face's voxels take the new vector
+        for (int a = 0; a < (face < 2 ? ny : nx); ++a)
+            for (int b = 0; b < (face < 4 ? nz : ny); ++b) {
+                const int i = face < 2 ? (face == 0 ? 0 : nx - 1) : a;
+                const int j = face < 2 ? a : (face < 4 ? (face == 2 ? 0 : ny - 1) : b);
+                const int k = face < 4 ? b : (face == 4 ? 0 : nz - 1);
+                const int v = mesh.voxel_index(i, j, k);
+                if (!dirichlet_planes.empty())
+                    for (int d = 0; d < number_of_densities(); ++d) write_dirichlet_value(v, d, value[d]);
+                if (!dirichlet_flags.empty()) dirichlet_flags[v] = 1;
+            }
+    }
     state_dirty = true;
+}
+
+// Face planes of the C ABI (bfa_set_dirichlet_planes) from the registered
+// faces, replayed in registration order xmin..zmax over the default 1.0.
+void Microenvironment::ensure_planes() {
+    if (!dirichlet_planes.empty()) return;
+    const int nx = mesh.x_nodes, ny = mesh.y_nodes, nz = mesh.z_nodes, ns = number_of_densities();
+    dirichlet_planes.assign((size_t)2 * ((size_t)ny * nz + (size_t)nx * nz + (size_t)nx * ny) * ns, 1.0);
+    for (int f = 0; f < 6; ++f) {
+        if (!face_registered[f]) continue;
+        for (int a = 0; a < (f < 2 ? ny : nx); ++a)
+            for (int b = 0; b < (f < 4 ? nz : ny); ++b) {
+                const int i = f < 2 ? (f == 0 ? 0 : nx - 1) : a;
+                const int j = f < 2 ? a : (f < 4 ? (f == 2 ? 0 : ny - 1) : b);
+                const int k = f < 4 ? b : (f == 4 ? 0 : nz - 1);
+                for (int d = 0; d < ns; ++d)
+                    write_dirichlet_value(mesh.voxel_index(i, j, k), d, face_values[(size_t)f * ns + d]);
+            }
+    }
+}
+
+// One voxel's value into every face plane holding it (none for interior
+// voxels: the LOD solvers never read them).
+void Microenvironment::write_dirichlet_value(int v, int d, double value) {
+    const int nx = mesh.x_nodes, ny = mesh.y_nodes, nz = mesh.z_nodes, ns = number_of_densities();
+    const int k = v % nz, j = (v / nz) % ny, i = v / (nz * ny);
+    const size_t pyz = (size_t)ny * nz * ns, pxz = (size_t)nx * nz * ns, pxy = (size_t)nx * ny * ns;
+    double* P = dirichlet_planes.data();
+    if (i == 0) P[((size_t)j * nz + k) * ns + d] = value;
+    if (i == nx - 1) P[pyz + ((size_t)j * nz + k) * ns + d] = value;
+    if (j == 0) P[2 * pyz + ((size_t)i * nz + k) * ns + d] = value;
+    if (j == ny - 1) P[2 * pyz + pxz + ((size_t)i * nz + k) * ns + d] = value;
+    if (k == 0) P[2 * (pyz + pxz) + ((size_t)i * ny + j) * ns + d] = value;
+    if (k == nz - 1) P[2 * (pyz + pxz) + pxy + ((size_t)i * ny + j) * ns + d] = value;
+}
+
+void Microenvironment::ensure_flags() {
+    if (!dirichlet_flags.empty()) return;
+    const int nx = mesh.x_nodes, ny = mesh.y_nodes, nz = mesh.z_nodes;
+    dirichlet_flags.assign(number_of_voxels(), 0);
+    for (int v = 0; v < number_of_voxels(); ++v) {
+        const int k = v % nz, j = (v / nz) % ny, i = v / (nz * ny);
+        dirichlet_flags[v] = (face_registered[0] && i == 0) || (face_registered[1] && i == nx - 1) ||
+                             (face_registered[2] && j == 0) || (face_registered[3] && j == ny - 1) ||
+                             (face_registered[4] && k == 0) || (face_registered[5] && k == nz - 1);
+    }
+}
+
+void Microenvironment::add_dirichlet_node(int voxel_index, const std::vector<double>& value) {
+    // BioFVM/BioFVM_microenvironment.cpp:218-226
+    update_dirichlet_node(voxel_index, value);
+}
+
+void Microenvironment::update_dirichlet_node(int voxel_index, const std::vector<double>& new_value) {
+    // BioFVM/BioFVM_microenvironment.cpp:228-235
+    if (voxel_index < 0 || voxel_index >= number_of_voxels() || (int)new_value.size() != number_of_densities()) {
+        std::cout << "Error: Dirichlet node needs a voxel index in range and one value per density" << std::endl;
+        return;
+    }
+    ensure_planes();
+    ensure_flags();
+    for (int d = 0; d < number_of_densities(); ++d) write_dirichlet_value(voxel_index, d, new_value[d]);
+    dirichlet_flags[voxel_index] = 1;
+    state_dirty = true;
+}
+
+void Microenvironment::update_dirichlet_node(int voxel_index, int substrate_index, double new_value) {
+    // BioFVM/BioFVM_microenvironment.cpp:237-245 (its per-voxel activation flag
+    // is not read by the LOD solvers)
+    if (voxel_index < 0 || voxel_index >= number_of_voxels() || substrate_index < 0 ||
+        substrate_index >= number_of_densities()) {
+        std::cout << "Error: Dirichlet node needs a voxel and a density index in range" << std::endl;
+        return;
+    }
+    ensure_planes();
+    ensure_flags();
+    write_dirichlet_value(voxel_index, substrate_index, new_value);
+    dirichlet_flags[voxel_index] = 1;
+    state_dirty = true;
+}
+
+void Microenvironment::remove_dirichlet_node(int voxel_index) {
+    // BioFVM/BioFVM_microenvironment.cpp:247-252: clears is_Dirichlet only; the
+    // face pass of the LOD solvers still writes the voxel's value vector
+    if (voxel_index < 0 || voxel_index >= number_of_voxels()) return;
+    ensure_flags();
+    dirichlet_flags[voxel_index] = 0;
+}
+
+bool Microenvironment::is_dirichlet_node(int voxel_index) const {
+    if (voxel_index < 0 || voxel_index >= number_of_voxels()) return false;
+    if (!dirichlet_flags.empty()) return dirichlet_flags[voxel_index] != 0;
+    const int nx = mesh.x_nodes, ny = mesh.y_nodes, nz = mesh.z_nodes;
+    const int k = voxel_index % nz, j = (voxel_index / nz) % ny, i = voxel_index / (nz * ny);
+    return (face_registered[0] && i == 0) || (face_registered[1] && i == nx - 1) ||
+           (face_registered[2] && j == 0) || (face_registered[3] && j == ny - 1) ||
+           (face_registered[4] && k == 0) || (face_registered[5] && k == nz - 1);
 }
 
 void Microenvironment::set_substrate_dirichlet_activation(int substrate_index, bool new_value) {
@@ -309,10 +444,42 @@ void Microenvironment::simulate_cell_sources_and_sinks(std::vector<Basic_Agent*>
         sig_c1.swap(c1);
         sig_c2.swap(c2);
         sig_e2.swap(e2);
+        tracking_on_device = false;  // the agents' own totals are sent again below
+    }
+    const bool track = track_internalized_substrates_in_each_agent && n > 0;
+    if (track && !tracking_on_device) {
+        // the agents hold the authoritative totals between calls
+        std::vector<double> tot((size_t)n * ns), e1((size_t)n * ns);
+        for (int a = 0; a < n; ++a) {
+            Basic_Agent* ag = agents[a];
+            ag->internalized_substrates.resize(ns, 0.0);
+            for (int d = 0; d < ns; ++d) {
+                tot[(size_t)a * ns + d] = ag->internalized_substrates[d];
+                e1[(size_t)a * ns + d] = ag->cell_source_sink_solver_temp_export1[d];
+            }
+        }
+        if (bfa_track_internalized(env, 1, tot.data(), e1.data())) {
+            std::cout << "Error: internalized tracking: " << bfa_last_error(env) << std::endl;
+            return;
+        }
+        tracking_on_device = true;
+    } else if (!track && tracking_on_device) {
+        (void)bfa_track_internalized(env, 0, nullptr, nullptr);
+        tracking_on_device = false;
     }
     if (bfa_secretion_uptake(env)) {
         std::cout << "Error: secretion/uptake: " << bfa_last_error(env) << std::endl;
         return;
+    }
+    if (track) {
+        std::vector<double> tot((size_t)n * ns);
+        if (bfa_get_internalized(env, tot.data())) {
+            std::cout << "Error: internalized tracking: " << bfa_last_error(env) << std::endl;
+            return;
+        }
+        for (int a = 0; a < n; ++a)
+            std::copy(tot.begin() + (size_t)a * ns, tot.begin() + (size_t)(a + 1) * ns,
+                      agents[a]->internalized_substrates.begin());
     }
     device_newer = true;
     gradients_valid = false;
@@ -607,24 +774,35 @@ bool load_initial_conditions_from_csv(Microenvironment& M, const std::string& fi
     return true;
 }
 
-void diffusion_decay_solver__constant_coefficients_LOD_3D_GPU(Microenvironment& M, double dt) {
-    // the reference solver needs a uniform Cartesian mesh (BioFVM/BioFVM_solvers.cpp:116-121)
-    if (!M.mesh.uniform_mesh) {
+int Microenvironment::step_with(int variant, double dt) {
+    // the reference solvers need a uniform Cartesian mesh (BioFVM/BioFVM_solvers.cpp:116-121)
+    if (!mesh.uniform_mesh) {
         std::cout << "Error: This algorithm is written for uniform Cartesian meshes. Try: other solvers!"
                   << std::endl
                   << std::endl;
-        return;
+        return BFA_ERR_ARG;
     }
-    if (M.push_state()) {
-        std::cout << "Error: diffusion solver setup failed: " << M.last_error << std::endl;
-        return;
+    if (push_state()) {
+        std::cout << "Error: diffusion solver setup failed: " << last_error << std::endl;
+        return BFA_ERR_STATE;
     }
-    if (bfa_diffusion_decay(M.env, dt)) {
-        std::cout << "Error: diffusion solver failed: " << bfa_last_error(M.env) << std::endl;
-        return;
+    int rc = bfa_set_solver(env, variant);
+    if (rc == BFA_OK) rc = bfa_diffusion_decay(env, dt);
+    if (rc) {
+        std::cout << "Error: diffusion solver failed: " << bfa_last_error(env) << std::endl;
+        return rc;
     }
-    M.device_newer = true;
-    M.gradients_valid = false;
+    device_newer = true;
+    gradients_valid = false;
+    return BFA_OK;
+}
+
+void diffusion_decay_solver__constant_coefficients_LOD_3D_GPU(Microenvironment& M, double dt) {
+    (void)M.step_with(BFA_LOD_3D_BLOCKING, dt);
+}
+
+void diffusion_decay_solver__constant_coefficients_LOD_3D_AVX256D_GPU(Microenvironment& M, double dt) {
+    (void)M.step_with(BFA_LOD_3D_AVX256D, dt);
 }
 
 }  // namespace BioFVM_amd

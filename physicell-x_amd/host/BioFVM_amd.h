@@ -67,6 +67,10 @@ class Basic_Agent {
     bool is_active = true;
     int current_voxel_index = -1;
     std::vector<double> secretion_rates, uptake_rates, saturation_densities, net_export_rates;
+    // substrate taken up minus substrate released, per density, when the
+    // microenvironment tracks it (track_internalized_substrates_in_each_agent;
+    // the reference keeps it behind Basic_Agent::internalized_substrates)
+    std::vector<double> internalized_substrates;
     // constants of the implicit source/sink update (BioFVM/BioFVM_basic_agent.cpp:241-265)
     std::vector<double> cell_source_sink_solver_temp1, cell_source_sink_solver_temp2;
     std::vector<double> cell_source_sink_solver_temp_export1, cell_source_sink_solver_temp_export2;
@@ -115,10 +119,25 @@ class Microenvironment {
     double* density_vector(int i, int j, int k) { return density_vector(mesh.voxel_index(i, j, k)); }
     std::vector<double>& density_field();  // whole field (host copy, synchronised)
 
-    // Dirichlet state (BioFVM/BioFVM_microenvironment.cpp:218-273,632)
-    void add_dirichlet_face(int face, const std::vector<double>& value);  // face 0..5 = xmin..zmax
+    // Dirichlet state (BioFVM/BioFVM_microenvironment.cpp:218-273,632).  Every
+    // voxel holds one value vector (dirichlet_value_vectors, default 1.0),
+    // written by the last call that covered it; the LOD solvers read it on the
+    // six faces only, so this keeps the face voxels' values.  add_dirichlet_face
+    // is add_dirichlet_node over every voxel of a face (xmin..zmax = 0..5), as
+    // initialize_microenvironment registers faces (:2649-2786).
+    void add_dirichlet_face(int face, const std::vector<double>& value);
+    void add_dirichlet_node(int voxel_index, const std::vector<double>& value);
+    void update_dirichlet_node(int voxel_index, const std::vector<double>& new_value);
+    void update_dirichlet_node(int voxel_index, int substrate_index, double new_value);
+    void remove_dirichlet_node(int voxel_index);
+    bool is_dirichlet_node(int voxel_index) const;
     void set_substrate_dirichlet_activation(int substrate_index, bool new_value);
     void apply_dirichlet_boundaries_conditions();
+
+    // default_microenvironment_options.track_internalized_substrates_in_each_agent
+    // (BioFVM/BioFVM_basic_agent.cpp:520-532,553-556): agents' internalized_substrates
+    // follow every simulate_cell_sources_and_sinks
+    bool track_internalized_substrates_in_each_agent = false;
 
     // bulk sources and sinks (BioFVM/BioFVM_microenvironment.cpp:1327-1368):
     // the three rate callbacks (zero_function by default, :167-169) are
@@ -155,7 +174,12 @@ class Microenvironment {
 
   private:
     friend void diffusion_decay_solver__constant_coefficients_LOD_3D_GPU(Microenvironment&, double);
+    friend void diffusion_decay_solver__constant_coefficients_LOD_3D_AVX256D_GPU(Microenvironment&, double);
     int ensure_device();
+    int step_with(int variant, double dt);
+    void ensure_planes();
+    void ensure_flags();
+    void write_dirichlet_value(int voxel_index, int substrate_index, double value);
     int push_state();
     std::vector<double> host_field;
     std::vector<double> gradient_field;  // [voxel][ns][3]
@@ -166,6 +190,12 @@ class Microenvironment {
     bool state_dirty = true;   // coefficients / Dirichlet changed
     unsigned char face_registered[6] = {0, 0, 0, 0, 0, 0};
     std::vector<double> face_values;  // [6][ns]
+    // per-voxel face values in the layout of bfa_set_dirichlet_planes, built
+    // from the faces on the first per-node call (empty: the faces describe
+    // every boundary voxel)
+    std::vector<double> dirichlet_planes;
+    std::vector<uint8_t> dirichlet_flags;  // is_Dirichlet per voxel, once any is set
+    bool tracking_on_device = false;
     // agent state last sent to the device
     std::vector<const Basic_Agent*> agents_sig;
     std::vector<int32_t> sig_voxel;
@@ -201,9 +231,13 @@ bool load_initial_conditions_from_matlab(Microenvironment& M, const std::string&
 // describe.)
 bool load_initial_conditions_from_csv(Microenvironment& M, const std::string& filename);
 
-// The drop-in solver: assign it to Microenvironment::diffusion_decay_solver
+// The drop-in solvers: assign one to Microenvironment::diffusion_decay_solver
 // (the reference assigns diffusion_decay_solver__constant_coefficients_LOD_3D
-// or _LOD_3D_BLOCKING at BioFVM/BioFVM_microenvironment.cpp:1943,2506).
+// or _LOD_3D_BLOCKING at BioFVM/BioFVM_microenvironment.cpp:1943,2506, and the
+// constructor defaults to _LOD_3D_AVX256D, :143).  _GPU reproduces
+// LOD_3D_BLOCKING (Dirichlet after the z sweep too), _AVX256D_GPU the AVX256D
+// solver (no Dirichlet pass after the z sweep).
 void diffusion_decay_solver__constant_coefficients_LOD_3D_GPU(Microenvironment& M, double dt);
+void diffusion_decay_solver__constant_coefficients_LOD_3D_AVX256D_GPU(Microenvironment& M, double dt);
 
 }  // namespace BioFVM_amd

@@ -13,7 +13,13 @@
 // Case file (little endian): int32 {nx, ny, nz, ns, steps, n_agents,
 // precision}; float64 {dx, dt, D[ns], decay[ns], active[ns], registered[6],
 // face_values[6*ns], field[nx*ny*nz*ns], then per agent: position[3],
-// volume, secretion[ns], uptake[ns], export[ns], saturation[ns]}.
+// volume, secretion[ns], uptake[ns], export[ns], saturation[ns]}; then
+// optionally int32 {n_nodes, n_single, solver (0 BLOCKING, 1 AVX256D),
+// track_internalized} and float64 {per node: voxel, values[ns]; per single
+// update: voxel, substrate, value}: Dirichlet nodes added after the faces
+// (add_dirichlet_node, then update_dirichlet_node for one substrate).  With
+// tracking on, the agents' internalized_substrates [n_agents][ns] follow the
+// other outputs.
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -90,6 +96,21 @@ int main(int argc, char** argv) {
         ag.update_voxel_index();
         list.push_back(&ag);
     }
+    int32_t extra[4] = {0, 0, 0, 0};
+    in.read(reinterpret_cast<char*>(extra), sizeof extra);
+    if (!in) extra[0] = extra[1] = extra[2] = extra[3] = 0;
+    in.clear();
+    for (int q = 0; q < extra[0]; ++q) {
+        const std::vector<double> nd = rd(1 + (size_t)ns);
+        M.add_dirichlet_node((int)nd[0], std::vector<double>(nd.begin() + 1, nd.end()));
+    }
+    for (int q = 0; q < extra[1]; ++q) {
+        const std::vector<double> up = rd(3);
+        M.update_dirichlet_node((int)up[0], (int)up[1], up[2]);
+    }
+    if (extra[2] == 1) M.diffusion_decay_solver = diffusion_decay_solver__constant_coefficients_LOD_3D_AVX256D_GPU;
+    M.track_internalized_substrates_in_each_agent = extra[3] != 0;
+
     std::vector<double> masses;
     for (int s = 0; s < steps; ++s) {
         M.simulate_diffusion_decay(dt);
@@ -113,6 +134,9 @@ int main(int argc, char** argv) {
     o.write(reinterpret_cast<const char*>(masses.data()), masses.size() * sizeof(double));
     o.write(reinterpret_cast<const char*>(grad.data()), grad.size() * sizeof(double));
     o.write(reinterpret_cast<const char*>(bulk.data()), bulk.size() * sizeof(double));
+    if (extra[3])
+        for (const Basic_Agent& ag : agents)
+            o.write(reinterpret_cast<const char*>(ag.internalized_substrates.data()), ns * sizeof(double));
     std::cout << "adapter_demo: " << steps << " steps of " << nx << "x" << ny << "x" << nz << "x" << ns
               << " with " << na << " agents done" << std::endl;
     return 0;

@@ -92,3 +92,98 @@ def test_cpp_host_side_matches_oracle(tmp_path, precision, tol):
         assert np.array_equal(bulk, bref)
     else:
         assert relmax(bulk, bref) <= 1e-6
+
+
+def write_extra(path, ns, nodes, singles, solver, track):
+    with open(path, "ab") as f:
+        np.array([len(nodes), len(singles), solver, track], np.int32).tofile(f)
+        for v, vals in nodes:
+            np.concatenate([[float(v)], vals]).astype(np.float64).tofile(f)
+        for v, d, val in singles:
+            np.array([v, d, val], np.float64).tofile(f)
+
+
+def planes_with_nodes(nx, ny, nz, ns, reg, fv, nodes, singles):
+    """Face planes after the faces (registration order) and then the nodes,
+    each voxel keeping the last value written to it."""
+    planes = oracle.dirichlet_planes(nx, ny, nz, ns, reg, fv)
+
+    def put(v, d, val):
+        i, j, k = v // (ny * nz), (v // nz) % ny, v % nz
+        if i == 0:
+            planes[0][j, k, d] = val
+        if i == nx - 1:
+            planes[1][j, k, d] = val
+        if j == 0:
+            planes[2][i, k, d] = val
+        if j == ny - 1:
+            planes[3][i, k, d] = val
+        if k == 0:
+            planes[4][i, j, d] = val
+        if k == nz - 1:
+            planes[5][i, j, d] = val
+
+    for v, vals in nodes:
+        for d in range(ns):
+            put(v, d, vals[d])
+    for v, d, val in singles:
+        put(v, d, val)
+    return planes
+
+
+@pytest.mark.parametrize("precision,tol", [(0, 1e-12), (1, 1e-5)])
+def test_cpp_host_side_dirichlet_nodes_avx256d_and_tracking(tmp_path, precision, tol):
+    """add_dirichlet_node / update_dirichlet_node on faces, edges and inside the
+    domain, the AVX256D solver pointer and internalised-substrate tracking,
+    through the C++ mirror, against the oracle."""
+    assert os.path.exists(DEMO), "build the C++ host side first (make -C physicell-x_amd/host)"
+    rng = np.random.default_rng(37)
+    nx, ny, nz, ns, steps, dx, dt = 21, 18, 25, 2, 4, 20.0, 0.01
+    D = 10 ** rng.uniform(3, 5, ns)
+    decay = rng.uniform(0.01, 1.0, ns)
+    active = np.array([1, 1], np.uint8)
+    reg = np.array([1, 1, 1, 1, 1, 1], np.uint8)
+    fv = rng.uniform(0, 40, (6, ns))
+    field = (fv[0] + rng.standard_normal((nx, ny, nz, ns))).ravel()
+
+    def vid(i, j, k):
+        return (i * ny + j) * nz + k
+
+    nodes = [(vid(0, 5, 7), [70.0, 3.0]), (vid(nx - 1, 0, 4), [11.0, 12.0]),   # face, edge
+             (vid(4, 6, nz - 1), [22.0, 33.0]), (vid(5, 5, 5), [99.0, 99.0]),  # face, interior (never read)
+             (vid(0, 0, 0), [1.5, 2.5])]                                         # corner
+    singles = [(vid(3, ny - 1, 9), 1, 44.0), (vid(0, 5, 7), 0, 71.0)]
+    na = 200
+    half = 0.5 * dx * np.array([nx, ny, nz])
+    pos = rng.uniform(-half, half, (na, 3))
+    pos[5] = pos[6] = pos[7]
+    agents = dict(pos=pos, V=rng.uniform(1000, 4000, na), S=rng.uniform(0, 10, (na, ns)),
+                  U=rng.uniform(0, 10, (na, ns)), E=rng.uniform(0, 20, (na, ns)),
+                  rs=rng.uniform(0, 40, (na, ns)))
+    case = tmp_path / "case.bin"
+    out = tmp_path / "out.bin"
+    write_case(case, nx, ny, nz, ns, steps, dx, dt, D, decay, active, reg, fv, field, agents, precision)
+    write_extra(case, ns, nodes, singles, solver=1, track=1)
+    r = subprocess.run([DEMO, str(case), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    res = np.fromfile(out, np.float64)
+    nf = field.size
+    got = res[:nf]
+    internal_got = res[-na * ns:].reshape(na, ns)
+    bbox = [-half[0], -half[1], -half[2], half[0], half[1], half[2]]
+    vox = oracle.voxel_indices(pos, bbox, dx, dx, dx, nx, ny, nz)
+    c1, c2, e1, e2 = oracle.agent_constants(dt, dx ** 3, agents["V"], agents["S"], agents["U"], agents["E"],
+                                            agents["rs"])
+    act = (vox >= 0).astype(np.uint8)
+    ag = oracle.AgentSet(np.where(vox >= 0, vox, 0), c1, c2, e1, e2, act)
+    orc = oracle.Oracle(nx, ny, nz, ns, dx, D, decay, active, reg, fv, solver="avx256d")
+    orc.set_planes(planes_with_nodes(nx, ny, nz, ns, reg, fv, nodes, singles))
+    u = field.copy()
+    internal = np.zeros((na, ns))
+    for s in range(steps):
+        orc.diffusion_step(u, dt)
+        orc.secretion_uptake(u, ag, internal)
+    g, uu = got.reshape(-1, ns), u.reshape(-1, ns)
+    assert max(relmax(g[:, d], uu[:, d]) for d in range(ns)) <= tol
+    assert relmax(internal_got, internal) <= tol
+    assert np.max(np.abs(internal)) > 1.0

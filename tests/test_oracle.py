@@ -35,9 +35,10 @@ def axis_matrix(n, k, lp):
     return A
 
 
-def dense_lod(u, nx, ny, nz, ns, D, decay, dt, dx, active=None, table=None):
+def dense_lod(u, nx, ny, nz, ns, D, decay, dt, dx, active=None, table=None, final=True):
     """One LOD step by dense solves; u is [nx, ny, nz, ns]; table is the per-voxel
-    Dirichlet value array [nx, ny, nz, ns] (only boundary entries are used)."""
+    Dirichlet value array [nx, ny, nz, ns] (only boundary entries are used);
+    final=False leaves out the Dirichlet pass after the last axis."""
     u = u.copy()
     bmask = np.zeros((nx, ny, nz), bool)
     bmask[[0, -1], :, :] = True
@@ -58,7 +59,8 @@ def dense_lod(u, nx, ny, nz, ns, D, decay, dt, dx, active=None, table=None):
             lp = decay[d] * dt / 3
             Ainv = np.linalg.inv(axis_matrix(n, k, lp))
             u[..., d] = np.moveaxis(np.tensordot(Ainv, np.moveaxis(u[..., d], axis, 0), 1), 0, axis)
-        dirichlet()
+        if axis < 2 or final:
+            dirichlet()
     return u
 
 
@@ -154,6 +156,102 @@ def test_full_step_with_dirichlet_against_dense_solve():
     orc.diffusion_step(got, dt)
     exp = dense_lod(u.reshape(nx, ny, nz, ns), nx, ny, nz, ns, D, decay, dt, dx, active, table).ravel()
     assert relmax(got, exp) < 1e-13
+
+
+def test_explicit_per_voxel_planes_against_dense_solve():
+    """Per-voxel boundary values (nodes overridden inside faces, on edges and
+    corners) given as planes reproduce a dense step with that value table."""
+    nx, ny, nz, ns = 6, 7, 5, 2
+    rng = np.random.default_rng(13)
+    D = 10 ** rng.uniform(3, 5, ns)
+    decay = rng.uniform(0.01, 1, ns)
+    dt, dx = 0.01, 20.0
+    active = np.array([1, 1], np.uint8)
+    table = rng.uniform(0, 40, (nx, ny, nz, ns))
+    t = table
+    planes = [t[0], t[-1], t[:, 0], t[:, -1], t[:, :, 0], t[:, :, -1]]
+    u = rng.uniform(0, 40, nx * ny * nz * ns)
+    orc = oracle.Oracle(nx, ny, nz, ns, dx, D, decay, active, np.ones(6, np.uint8), np.ones((6, ns)))
+    orc.set_planes(planes)
+    got = u.copy()
+    orc.diffusion_step(got, dt)
+    exp = dense_lod(u.reshape(nx, ny, nz, ns), nx, ny, nz, ns, D, decay, dt, dx, active, table).ravel()
+    assert relmax(got, exp) < 1e-13
+    # the face-table form is the special case of the same planes
+    planes2 = oracle.dirichlet_planes(nx, ny, nz, ns, np.ones(6, np.uint8), rng.uniform(0, 40, (6, ns)))
+    orc.set_planes(planes2)
+    a = u.copy()
+    orc.diffusion_step(a, dt)
+    assert relmax(a, got) > 1e-6
+
+
+def test_avx256d_variant_skips_the_last_dirichlet_pass():
+    nx, ny, nz, ns = 6, 5, 7, 2
+    rng = np.random.default_rng(14)
+    D = 10 ** rng.uniform(3, 5, ns)
+    decay = rng.uniform(0.01, 1, ns)
+    dt, dx = 0.01, 20.0
+    reg = np.array([1, 1, 0, 1, 1, 0], np.uint8)
+    fv = rng.uniform(0, 40, (6, ns))
+    active = np.array([1, 0], np.uint8)
+    table = np.ones((nx, ny, nz, ns))
+    sl = [np.s_[0, :, :], np.s_[-1, :, :], np.s_[:, 0, :], np.s_[:, -1, :], np.s_[:, :, 0], np.s_[:, :, -1]]
+    for f in range(6):
+        if reg[f]:
+            table[sl[f]] = fv[f]
+    u = rng.uniform(0, 40, nx * ny * nz * ns)
+    avx = oracle.Oracle(nx, ny, nz, ns, dx, D, decay, active, reg, fv, solver="avx256d")
+    got = u.copy()
+    avx.diffusion_step(got, dt)
+    exp = dense_lod(u.reshape(nx, ny, nz, ns), nx, ny, nz, ns, D, decay, dt, dx, active, table,
+                    final=False).ravel()
+    assert relmax(got, exp) < 1e-13
+    # and it differs from BLOCKING exactly on the active substrate's faces
+    blk = oracle.Oracle(nx, ny, nz, ns, dx, D, decay, active, reg, fv)
+    b = u.copy()
+    blk.diffusion_step(b, dt)
+    diff = np.abs(b - got).reshape(nx, ny, nz, ns)
+    assert np.max(diff[1:-1, 1:-1, 1:-1]) == 0.0 and np.max(diff[..., 1]) == 0.0
+    assert np.max(diff[..., 0]) > 1e-6
+
+
+def test_internalised_tracking_per_agent_matches_its_density_change():
+    """Each agent's internalised change is minus the substrate it moved:
+    -W * (density after its update - density before), the export included
+    (e2 = e1 / W), for agents sharing voxels, in index order."""
+    ns, dt, dx = 2, 0.01, 20.0
+    W = dx ** 3
+    rng = np.random.default_rng(15)
+    n = 12
+    voxel = np.array([3, 3, 5, 3, 7, 5, 1, 3, 9, 9, 2, 3], np.int32)
+    c1, c2, e1, e2 = oracle.agent_constants(dt, W, rng.uniform(1000, 4000, n), rng.uniform(0, 10, (n, ns)),
+                                            rng.uniform(0, 10, (n, ns)), rng.uniform(0, 20, (n, ns)),
+                                            rng.uniform(0, 40, (n, ns)))
+    u = rng.uniform(0, 40, 10 * ns)
+    internal = np.zeros((n, ns))
+    orc = oracle.Oracle(10, 1, 1, ns, dx, [0.0, 0.0], [0.0, 0.0])
+    for a in range(n):
+        one = oracle.AgentSet(voxel[a:a + 1], c1[a:a + 1], c2[a:a + 1], e1[a:a + 1], e2[a:a + 1])
+        before = u[voxel[a] * ns:(voxel[a] + 1) * ns].copy()
+        mine = np.zeros((1, ns))
+        orc.secretion_uptake(u, one, mine)
+        after = u[voxel[a] * ns:(voxel[a] + 1) * ns]
+        np.testing.assert_allclose(mine[0], -W * (after - before), rtol=1e-9, atol=1e-9)
+        internal[a] = mine[0]
+    # the whole set in one call gives the same per-agent totals
+    u2 = u.copy()
+    u2[:] = rng.uniform(0, 40, 10 * ns)
+    v = u2.copy()
+    tot = np.zeros((n, ns))
+    orc.secretion_uptake(v, oracle.AgentSet(voxel, c1, c2, e1, e2), tot)
+    seq = np.zeros((n, ns))
+    w = u2.copy()
+    for a in range(n):
+        one = oracle.AgentSet(voxel[a:a + 1], c1[a:a + 1], c2[a:a + 1], e1[a:a + 1], e2[a:a + 1])
+        m = np.zeros((1, ns))
+        orc.secretion_uptake(w, one, m)
+        seq[a] = m[0]
+    assert np.array_equal(tot, seq) and np.array_equal(v, w)
 
 
 def test_neumann_cosine_eigenmode_decay():
