@@ -454,23 +454,31 @@ def large_workload(n=800, seed=3, na=1_000_000):
                 S=S, U=U, E=E, rs=rs, V=V)
 
 
-def run_large(pkg, n=800, steps=20, seed=3):
-    """800^3 x 2 FP32 scaled-deviation form vs the FP64 path on the config-3
-    workload (large_workload)."""
+def run_large(pkg, n=800, steps=20, seed=3, tune=True):
+    """800^3 x 2 FP32 scaled-deviation form and FP64 path on the config-3
+    workload (large_workload): autotuned shapes, graph-replayed step time,
+    per-kernel times with the dominant kernel's roofline, and FP32 vs FP64
+    max relative error and mass drift per substrate after the run."""
+    import importlib
+    at = importlib.import_module("physicellx_amd.autotune")
     w = large_workload(n, seed)
     ns, D, decay, dval = w["ns"], w["D"], w["decay"], w["dval"]
     voxel, c1, c2, e2 = w["voxel"], w["c1"], w["c2"], w["e2"]
-    na = voxel.size
-    field = np.empty(ns * n ** 3)
-    field[: n ** 3] = 38.0
-    field[n ** 3:] = 0.0
-    out = {"grid": [n, n, n], "substrates": ns, "agents": na, "steps": steps}
+    updates = n ** 3 * ns
+    out = {"workload": f"{n}^3 voxels x {ns} substrates, {voxel.size} spheroid agents, dt=0.01 min",
+           "grid": [n, n, n], "substrates": ns, "agents": int(voxel.size), "steps": steps}
     envs = {}
     for prec in ("fp32", "fp64"):
+        tiles = at.autotune(pkg, n, n, n, ns, prec, 0.01) if tune else None
         e = pkg.Microenvironment(n, n, n, ns, 20.0, prec)
+        if tiles is not None:
+            e.set_tiles(tiles["Lx"], tiles["Ly"], tiles["Lz"], tiles["rows_x"], tiles["cols_y"], tiles["cols_z"])
+            e.set_x_grid(tiles.get("x_grid", 0))
+            e.set_slab(tiles.get("slab", 0))
+            e.set_substrate_streams(tiles.get("substrate_streams", 1))
         e.set_coefficients(D, decay)
         e.set_dirichlet(np.ones(ns, np.uint8), np.ones(6, np.uint8), np.tile(dval, (6, 1)))
-        e.upload(field, "soa")
+        e.fill(dval)
         e.set_agents(voxel, c1, c2, e2)
         e.step(0.01, 1)
         e.synchronize()
@@ -478,15 +486,29 @@ def run_large(pkg, n=800, steps=20, seed=3):
         e.step(0.01, steps - 1)
         e.synchronize()
         dt_ = time.perf_counter() - t0
-        out[prec] = {"value": n ** 3 * ns * (steps - 1) / dt_, "ms_per_step": 1e3 * dt_ / (steps - 1),
-                     "tiles": list(e.tiles())}
+        ms = e.time_kernels(0.01, 5)
+        tsize = 4 if prec == "fp32" else 8
+        names = ["x", "y", "z"]
+        dom = int(np.argmax(ms[:3]))
+        alg = updates * 2 * tsize
+        achieved = alg / (ms[dom] * 1e-3) / 1e9
+        out[prec] = {"value": updates * (steps - 1) / dt_, "ms_per_step": 1e3 * dt_ / (steps - 1),
+                     "step_alg_frac_of_peak": updates * 6 * tsize / (dt_ / (steps - 1)) / 1e9 / PEAK_HBM_GBS,
+                     "kernel_ms": dict(x=ms[0], y=ms[1], z=ms[2], agents=ms[3]),
+                     "roofline": {"bound": "hbm", "kernel": f"sweep_{names[dom]}", "alg_bytes_per_launch": alg,
+                                  "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                  "frac": achieved / PEAK_HBM_GBS,
+                                  "traffic": load_traffic(f"sweep_{names[dom]}_{prec}_{n}")},
+                     "tiles": list(e.tiles()), "autotuned": tune}
         envs[prec] = e
     f32 = envs["fp32"].download("soa").reshape(ns, -1)
     f64 = envs["fp64"].download("soa").reshape(ns, -1)
     m32, m64 = envs["fp32"].total_mass(), envs["fp64"].total_mass()
-    out["max_rel_err_per_substrate"] = [float(np.max(np.abs(f32[d] - f64[d])) / np.max(np.abs(f64[d])))
-                                        for d in range(ns)]
-    out["mass_drift_per_substrate"] = [float(abs(m32[d] - m64[d]) / abs(m64[d])) for d in range(ns)]
+    out["fp32_vs_fp64"] = {
+        "max_rel_err_per_substrate": [float(np.max(np.abs(f32[d] - f64[d])) / np.max(np.abs(f64[d])))
+                                      for d in range(ns)],
+        "mass_drift_per_substrate": [float(abs(m32[d] - m64[d]) / abs(m64[d])) for d in range(ns)],
+        "steps": steps + 5, "norm": "max|f32 - f64| / max|f64| per substrate"}
     for e in envs.values():
         e.close()
     return out
@@ -616,6 +638,7 @@ def main():
     }
     out.update(res)
     if args.mode == "default":
+        out["large"] = run_large(pkg)
         ens = run_ensemble(pkg, repeats=3)
         srv = run_serving(pkg, n_req=args.requests)
         out["batched_solve"] = {"p50_s": ens["fp32"]["p50_s"], "p99_s": ens["fp32"]["p99_s"],

@@ -800,7 +800,7 @@ int bfa_create(bfa_env** out, int nx, int ny, int nz, int ns, double dx, int pre
             (rc = dalloc(e, &e->invT[a], n * e->tsize)) || (rc = dalloc(e, &e->cpT[a], n * e->tsize)))
             return bail(rc);
     }
-    if ((rc = dalloc(e, &e->kT, ns * e->tsize)) || (rc = dalloc(e, &e->betaT, ns * e->tsize)) ||
+    if ((rc = dalloc(e, &e->kT, 2 * (size_t)ns * e->tsize)) || (rc = dalloc(e, &e->betaT, ns * e->tsize)) ||
         (rc = dalloc(e, &e->dvalT, 7 * (size_t)ns * e->tsize)) ||
         (rc = dalloc(e, &e->params, 5 * (size_t)ns * 8)))
         return bail(rc);

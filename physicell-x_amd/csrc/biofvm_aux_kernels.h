@@ -15,9 +15,10 @@ namespace bfa {
 //   c1 = D*dt/dx/dx, c2 = lambda*dt/3, c3 = 1+c1+c1+c2, c3a = 1+c1+c2,
 //   den[0] = c3a (1+c2 when n == 1), cp[0] = -c1/den[0],
 //   den[i] = (i == n-1 ? c3a : c3) + c1*cp[i-1], cp[i] = -c1/den[i].
-// Writes den/cp in FP64 and 1/den, cp, c1 in the sweep type T.  For the FP32
-// deviation form (u = b + s*v) it also writes beta = (b/s)*c2: one sweep maps
-// v to T^-1 (v - beta) because every axis matrix has unit row sums plus c2.
+// Writes den/cp in FP64 and 1/den, cp in the sweep type T, and kT[d] = c1,
+// kT[ns + d] = c2.  For the FP32 deviation form (u = b + s*v) it also writes
+// beta = (b/s)*c2: one sweep maps v to T^-1 (v - beta) because every axis
+// matrix has unit row sums plus c2.
 template <typename T>
 __global__ void k_thomas(int n, int ns, const double* __restrict__ params, double dt, double dx,
                          double* __restrict__ den64, double* __restrict__ cp64,
@@ -55,6 +56,7 @@ __global__ void k_thomas(int n, int ns, const double* __restrict__ params, doubl
         cpT[(size_t)d * n + i] = (T)cp[i];
     }
     kT[d] = (T)c1;
+    kT[ns + d] = (T)c2;
     betaT[d] = (T)((b / s) * c2);
 }
 

@@ -170,6 +170,7 @@ __global__ __launch_bounds__(kb_col_max_threads(L * sizeof(T))) void kb_sweep_co
         const int s = p * L;
         const T bt = DEV ? a.beta[d] : T(0);
         const T kd = a.k[d];
+        const T lam = DEV ? a.k[a.ns + d] : T(0);
         const bool dact = (a.active >> d) & 1ull;
         // column c of the item: its (ci, co), offset of its first node, and
         // whether this thread holds part of it
@@ -181,30 +182,43 @@ __global__ __launch_bounds__(kb_col_max_threads(L * sizeof(T))) void kb_sweep_co
             return on;
         };
         T xn[L];
+        T hl_n = T(0), hr_n = T(0);  // FP32 delta form: the chunk's outer neighbours
         int ci_n, co_n;
         unsigned base_n;
         bool act_n = col_of(w.c0 + lane, ci_n, co_n, base_n);
-        load_line_chunk<T, L>(xn, ub + (size_t)base_n + (size_t)s * stride, stride, act_n ? n - s : 0);
+        auto fetch = [&](bool on, unsigned bse) {
+            const T* line = ub + (size_t)bse + (size_t)s * stride;
+            load_line_chunk<T, L>(xn, line, stride, on ? n - s : 0);
+            if (DEV) {
+                hl_n = (on && s > 0) ? *(line - stride) : T(0);
+                hr_n = (on && s + L < n) ? line[(size_t)L * stride] : T(0);
+            }
+        };
+        fetch(act_n, base_n);
         if (t == 0) s_item[slot ^ 1] = nxt;
         for (int q = 0; q < w.ntiles; ++q) {
             T x[L];
 #pragma unroll
             for (int m = 0; m < L; ++m) x[m] = xn[m];
+            const T hl = hl_n, hr = hr_n;
             const unsigned base = base_n;
             const int ci = ci_n, co = co_n;
             const bool act = act_n;
             if (q + 1 < w.ntiles) {
                 act_n = col_of(w.c0 + (q + 1) * W + lane, ci_n, co_n, base_n);
-                load_line_chunk<T, L>(xn, ub + (size_t)base_n + (size_t)s * stride, stride,
-                                      act_n ? n - s : 0);
+                fetch(act_n, base_n);
             }
             if (DEV) {
-                // padded nodes (past the line end) do not influence the line
+                T r[L];
+                delta_residual<T, L>(x, hl, hr, s, act ? n : 0, kd, lam, bt, r);
+                __syncthreads();  // staging done / the previous tile's fold readers are done
+                chunk_solve<T, L>(r, act, p, P, lane, W, s_inv, s_cp, kd, s_z, s_g);
 #pragma unroll
-                for (int m = 0; m < L; ++m) x[m] = x[m] - bt;
+                for (int m = 0; m < L; ++m) x[m] = x[m] + r[m];
+            } else {
+                __syncthreads();  // staging done / the previous tile's fold readers are done
+                chunk_solve<T, L>(x, act, p, P, lane, W, s_inv, s_cp, kd, s_z, s_g);
             }
-            __syncthreads();  // staging done / the previous tile's fold readers are done
-            chunk_solve<T, L>(x, act, p, P, lane, W, s_inv, s_cp, kd, s_z, s_g);
             if (act && dact) col_dirichlet<T, L, AXIS>(x, a, d, ci, co, s, n, p, P);
             store_line_chunk<T, L>(x, ub + (size_t)base + (size_t)s * stride, stride, act ? n - s : 0);
         }
@@ -291,6 +305,7 @@ __global__ __launch_bounds__(XYShape<T>::threads) void kb_sweep_xy(
                        t, nthr);
         const T bt = DEV ? ay.beta[d] : T(0);
         const T ky = ay.k[d];
+        const T lam = DEV ? ay.k[ay.ns + d] : T(0);
         const bool dact = (ay.active >> d) & 1ull;
         if (t == 0) s_item[slot ^ 1] = nxt;
         // the plane's 16-byte vectors v = t + q*nthr, q < NV; the last one may
@@ -352,10 +367,23 @@ __global__ __launch_bounds__(XYShape<T>::threads) void kb_sweep_xy(
                 const bool act = py < PY && i < nx;
                 T x[LY];
                 const T* col = plane + sy * nx + i;
+                if (DEV) {
 #pragma unroll
-                for (int m = 0; m < LY; ++m) x[m] = (act && sy + m < ny) ? col[m * nx] - bt : T(0);
-                __syncthreads();  // the previous tile's fold readers are done
-                chunk_solve<T, LY>(x, act, py, PY, ly, W, s_inv, s_cp, ky, s_z, s_g);
+                    for (int m = 0; m < LY; ++m) x[m] = (act && sy + m < ny) ? col[m * nx] : T(0);
+                    const T hl = (act && sy > 0) ? col[-nx] : T(0);
+                    const T hr = (act && sy + LY < ny) ? col[LY * nx] : T(0);
+                    T r[LY];
+                    delta_residual<T, LY>(x, hl, hr, sy, act ? ny : 0, ky, lam, bt, r);
+                    __syncthreads();  // the previous tile's fold readers are done
+                    chunk_solve<T, LY>(r, act, py, PY, ly, W, s_inv, s_cp, ky, s_z, s_g);
+#pragma unroll
+                    for (int m = 0; m < LY; ++m) x[m] = x[m] + r[m];
+                } else {
+#pragma unroll
+                    for (int m = 0; m < LY; ++m) x[m] = (act && sy + m < ny) ? col[m * nx] : T(0);
+                    __syncthreads();  // the previous tile's fold readers are done
+                    chunk_solve<T, LY>(x, act, py, PY, ly, W, s_inv, s_cp, ky, s_z, s_g);
+                }
                 if (act && dact) col_dirichlet<T, LY, 1>(x, ay, d, i, k, sy, ny, py, PY);
                 if (act) {
                     T* cw = plane + sy * nx + i;

@@ -53,7 +53,7 @@ struct SweepArgs {
     T* u;             // field base of this microenvironment (SoA)
     const T* inv;     // [ns][n] 1/den for the swept axis
     const T* cp;      // [ns][n] c' for the swept axis
-    const T* k;       // [ns] D*dt/dx^2
+    const T* k;       // [2][ns]: D*dt/dx^2, then decay*dt/3
     const T* beta;    // [ns] deviation shift (FP32 form), unused for FP64
     const T* dval;    // [7][ns] Dirichlet value per face slot
     // Per-voxel Dirichlet values of the six face planes, or nullptr to use
@@ -206,6 +206,37 @@ __device__ __forceinline__ void stage_coefs(T* s_inv, T* s_cp, const T* __restri
     }
 }
 
+// ---- FP32 scaled-deviation form: the line solve in delta form --------------
+// One sweep maps v to x = T^-1 (v - beta).  Written as x = v + delta with
+//   T delta = r,  r = (v - beta) - T v
+//     r_i = -beta - lam*v_i + k*((v_{i-1} - v_i) + (v_{i+1} - v_i))
+// (no neighbour term past either end of the line: Neumann rows), the FP32
+// arithmetic acts on the small residual and increment instead of the values
+// themselves.  The plain form rounds T^-1 (v - beta) in FP32 with
+// FP32-rounded coefficients, which perturbs the operator's decay factor by a
+// few units in the last place; near a steady state that shifts the fixed
+// point by about eps32 / (decay * dt) relative (1e-5 after a few hundred
+// steps at decay 0.35 /min).  In delta form the fixed point is where r = 0,
+// set by k, lam and beta alone, and the coefficient rounding only changes how
+// fast delta converges.
+// x[] holds the chunk's values at nodes s..s+L-1 of a line of n nodes, vl and
+// vr those at s-1 and s+L (ignored past the line ends); r[] gets the residual,
+// zero past the line end.
+template <typename T, int L>
+__device__ __forceinline__ void delta_residual(const T (&x)[L], T vl, T vr, int s, int n, T k, T lam,
+                                               T beta, T (&r)[L]) {
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+        const int pos = s + m;
+        const T left = m == 0 ? vl : x[m - 1];
+        const T right = m == L - 1 ? vr : x[m + 1];
+        T lap = T(0);
+        if (pos > 0) lap = left - x[m];
+        if (pos < n - 1) lap = lap + (right - x[m]);
+        r[m] = pos < n ? fma(k, lap, fma(-lam, x[m], -beta)) : T(0);
+    }
+}
+
 // ---- x sweep: R rows per block staged through LDS ---------------------------
 // blockDim.x = roundup(R*P, 64) with P = ceil(nx/L); grid = (ceil(ny*nz/R), ns).
 // Dynamic LDS: (R*S + 2*P*L + P*R + P) * sizeof(T), S = nx | 1.
@@ -264,11 +295,23 @@ __global__ __launch_bounds__(1024) void k_sweep_x(SweepArgs<T> a, int R, int log
         }
     }
     if (DEV) {
-        const T bt = a.beta[d];
+        // the neighbours of the chunk's end nodes, after the Dirichlet pass
+        if (act) {
 #pragma unroll
-        for (int m = 0; m < L; ++m) x[m] = (s + m < nx) ? x[m] - bt : T(0);
+            for (int m = 0; m < L; ++m)
+                if (s + m < nx) tile[rho * S + s + m] = x[m];
+        }
+        __syncthreads();
+        const T vl = (act && s > 0) ? tile[rho * S + s - 1] : T(0);
+        const T vr = (act && s + L < nx) ? tile[rho * S + s + L] : T(0);
+        T r[L];
+        delta_residual<T, L>(x, vl, vr, s, act ? nx : 0, a.k[d], a.k[a.ns + d], a.beta[d], r);
+        chunk_solve<T, L>(r, act, p, P, rho, R, s_inv, s_cp, a.k[d], s_z, s_g);
+#pragma unroll
+        for (int m = 0; m < L; ++m) x[m] = x[m] + r[m];
+    } else {
+        chunk_solve<T, L>(x, act, p, P, rho, R, s_inv, s_cp, a.k[d], s_z, s_g);
     }
-    chunk_solve<T, L>(x, act, p, P, rho, R, s_inv, s_cp, a.k[d], s_z, s_g);
     if (act && dact) {
 #pragma unroll
         for (int m = 0; m < L; ++m) {
@@ -404,7 +447,7 @@ template <typename T, int L>
 struct XCoef {
     T inv[L], cp[L];  // 1/den and c' of the lane's nodes, zero past the row end
     T fm[6], bm[6];   // multiplier of forward / backward scan step j, 0 where the step is off
-    T k, bt;          // D*dt/dx^2 and the deviation shift
+    T k, bt, lam;     // D*dt/dx^2, the deviation shift and decay*dt/3
     T v_lo, v_hi;     // Dirichlet values at the x ends of rows on no y/z face
     bool dact;        // Dirichlet on for this substrate
     bool plane;       // per-voxel Dirichlet planes (values fetched per row)
@@ -421,6 +464,7 @@ __device__ __forceinline__ void x_coef_setup(const SweepArgs<T>& a, int d, int p
     }
     c.k = a.k[d];
     c.bt = DEV ? a.beta[d] : T(0);
+    c.lam = DEV ? a.k[a.ns + d] : T(0);
     c.dact = (a.active >> d) & 1ull;
     c.plane = a.plane != nullptr;
     c.v_lo = a.dval[dir_slot_xonly(0, nx, a.reg) * a.ns + d];
@@ -539,14 +583,20 @@ __device__ __forceinline__ void x_row_dirichlet(T (&cur)[L], int nx, int s, bool
 template <typename T, int L, bool DEV>
 __device__ __forceinline__ void x_row_compute(int nx, bool have, int p, int P, int s,
                                               const XCoef<T, L>& c, XRow<T, L>& q) {
-    T(&cur)[L] = q.v;
-    x_row_dirichlet<T, L>(cur, nx, s, have, c, q);
+    x_row_dirichlet<T, L>(q.v, nx, s, have, c, q);
+    // FP64: solve in place on the values.  FP32 (DEV): solve for the
+    // increment from the residual (delta_residual); the neighbours of the
+    // chunk's end nodes are the last node of lane p-1 and the first of lane
+    // p+1 (lanes of other rows are masked by the line-end tests).
+    T r[L];
+    if (DEV) {
+        const T vl = shfl_up_t(q.v[L - 1], 1);
+        const T vr = shfl_down_t(q.v[0], 1);
+        delta_residual<T, L>(q.v, vl, vr, s, nx, c.k, c.lam, c.bt, r);
+    }
+    T(&cur)[L] = DEV ? r : q.v;
     // past the row end 1/den = c' = 0, so whatever sits there leaves the
     // row's values unchanged and becomes 0 in the forward pass
-    if (DEV) {
-#pragma unroll
-        for (int m = 0; m < L; ++m) cur[m] = cur[m] - c.bt;
-    }
     T zb = T(0);
 #pragma unroll
     for (int m = 0; m < L; ++m) zb = fma(c.k, zb, cur[m]) * c.inv[m];
@@ -583,7 +633,11 @@ __device__ __forceinline__ void x_row_compute(int nx, bool have, int p, int P, i
         xn = fma(-c.cp[m], xn, cur[m]);
         cur[m] = xn;
     }
-    if (have) x_row_dirichlet<T, L>(cur, nx, s, have, c, q);
+    if (DEV) {
+#pragma unroll
+        for (int m = 0; m < L; ++m) q.v[m] = q.v[m] + r[m];
+    }
+    if (have) x_row_dirichlet<T, L>(q.v, nx, s, have, c, q);
 }
 
 // Store of a solved row group through LDS.  A row group (nrv rows of nx
@@ -851,13 +905,20 @@ __global__ __launch_bounds__(1024) void k_sweep_col(SweepArgs<T> a) {
     T x[L];
     load_line_chunk<T, L>(x, line, stride, cnt);
     if (DEV) {
-        // padded nodes (past the line end) do not influence the line
-        const T bt = a.beta[d];
+        // delta form: the chunk's outer neighbours come from the field (the
+        // neighbouring chunks' threads load them too, so they hit in cache)
+        const T vl = (act && s > 0) ? *(line - stride) : T(0);
+        const T vr = (act && s + L < n) ? line[(size_t)L * stride] : T(0);
+        T r[L];
+        delta_residual<T, L>(x, vl, vr, s, act ? n : 0, a.k[d], a.k[a.ns + d], a.beta[d], r);
+        __syncthreads();  // coefficient staging complete
+        chunk_solve<T, L>(r, act, p, P, lane, TX, s_inv, s_cp, a.k[d], s_z, s_g);
 #pragma unroll
-        for (int m = 0; m < L; ++m) x[m] = x[m] - bt;
+        for (int m = 0; m < L; ++m) x[m] = x[m] + r[m];
+    } else {
+        __syncthreads();  // coefficient staging complete
+        chunk_solve<T, L>(x, act, p, P, lane, TX, s_inv, s_cp, a.k[d], s_z, s_g);
     }
-    __syncthreads();  // coefficient staging complete
-    chunk_solve<T, L>(x, act, p, P, lane, TX, s_inv, s_cp, a.k[d], s_z, s_g);
     if (act && ((a.active >> d) & 1ull)) col_dirichlet<T, L, AXIS>(x, a, d, ci, co, s, n, p, P);
     store_line_chunk<T, L>(x, line, stride, cnt);
 }
