@@ -187,3 +187,75 @@ def test_cpp_host_side_dirichlet_nodes_avx256d_and_tracking(tmp_path, precision,
     assert max(relmax(g[:, d], uu[:, d]) for d in range(ns)) <= tol
     assert relmax(internal_got, internal) <= tol
     assert np.max(np.abs(internal)) > 1.0
+
+
+BINDING = os.path.join(ROOT, "tests", "cpp", "binding_demo")
+
+
+def test_reference_binding_templates_against_oracle(tmp_path):
+    """The reference-tree binding (BioFVM_reference_binding.h) instantiated on
+    stand-ins of the reference's public types: the MPI-signature solver at
+    world size 1, the container-level agent call with internalised tracking,
+    per-voxel Dirichlet nodes read through the face pass, and Dirichlet changes
+    between steps picked up by the next call."""
+    assert os.path.exists(BINDING), "build the C++ host side first (make -C physicell-x_amd/host)"
+    rng = np.random.default_rng(61)
+    nx, ny, nz, ns, steps, change, dx, dt = 19, 23, 17, 2, 5, 2, 20.0, 0.01
+    D = 10 ** rng.uniform(3, 5, ns)
+    decay = rng.uniform(0.01, 1.0, ns)
+    active = np.array([1, 1], np.uint8)
+    field = rng.uniform(0, 40, nx * ny * nz * ns)
+
+    def vid(i, j, k):
+        return (i * ny + j) * nz + k
+
+    # every face voxel gets its own value vector, then a few overrides
+    nodes = []
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                if i in (0, nx - 1) or j in (0, ny - 1) or k in (0, nz - 1):
+                    nodes.append((vid(i, j, k), rng.uniform(0, 40, ns)))
+    nodes.append((vid(0, 3, 4), np.array([77.0, 66.0])))
+    nodes.append((vid(5, 5, 5), np.array([99.0, 99.0])))  # interior: never read
+    chg = (vid(nx - 1, 7, 8), 0, 55.0, 1)  # (voxel, substrate, value) then substrate 1 switched off
+    na = 150
+    voxel = rng.integers(0, nx * ny * nz, na).astype(np.int32)
+    voxel[4] = voxel[5] = voxel[6]
+    act = np.ones(na, np.uint8)
+    act[9] = 0
+    c1, c2, e1, e2 = oracle.agent_constants(dt, dx ** 3, rng.uniform(1000, 4000, na), rng.uniform(0, 10, (na, ns)),
+                                            rng.uniform(0, 10, (na, ns)), rng.uniform(0, 20, (na, ns)),
+                                            rng.uniform(0, 40, (na, ns)))
+    case, out = tmp_path / "case.bin", tmp_path / "out.bin"
+    with open(case, "wb") as f:
+        np.array([nx, ny, nz, ns, steps, na, len(nodes), change], np.int32).tofile(f)
+        np.concatenate([[dx, dt], D, decay, active.astype(float), field]).astype(np.float64).tofile(f)
+        for v, vals in nodes:
+            np.concatenate([[float(v)], vals]).astype(np.float64).tofile(f)
+        np.array(chg, np.float64).tofile(f)
+        for a in range(na):
+            np.concatenate([[voxel[a], act[a]], c1[a], c2[a], e1[a], e2[a]]).astype(np.float64).tofile(f)
+    r = subprocess.run([BINDING, str(case), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "one GPU" in r.stdout  # the world-size-2 call printed its error
+    res = np.fromfile(out, np.float64)
+    got, internal_got = res[:field.size], res[field.size:].reshape(na, ns)
+    table = np.ones((nx, ny, nz, ns))
+    for v, vals in nodes:
+        table[v // (ny * nz), (v // nz) % ny, v % nz] = vals
+    orc = oracle.Oracle(nx, ny, nz, ns, dx, D, decay, active, np.ones(6, np.uint8), np.ones((6, ns)))
+    ag = oracle.AgentSet(voxel, c1, c2, e1, e2, act)
+    u = field.copy()
+    internal = np.zeros((na, ns))
+    t = table
+    for s in range(steps):
+        if s == change:
+            v, d, val, off = chg
+            t[v // (ny * nz), (v // nz) % ny, v % nz, d] = val
+            orc.active[off] = 0
+        orc.set_planes([t[0], t[-1], t[:, 0], t[:, -1], t[:, :, 0], t[:, :, -1]])
+        orc.diffusion_step(u, dt)
+        orc.secretion_uptake(u, ag, internal)
+    assert relmax(got, u) <= 1e-12
+    assert relmax(internal_got, internal) <= 1e-12
