@@ -142,6 +142,14 @@ enum bfa_solver { BFA_LOD_3D_BLOCKING = 0, BFA_LOD_3D_AVX256D = 1 };
 int bfa_set_solver(bfa_env *env, int variant);
 int bfa_get_solver(const bfa_env *env);
 
+/* Thomas coefficients across dt changes.  keep_first == 0 (the default)
+ * recomputes them whenever dt changes.  keep_first == 1 reproduces the
+ * reference: its solver computes them on the first call and keeps them for
+ * every later dt (diffusion_solver_setup_done, BioFVM/BioFVM_solvers.cpp:
+ * 124-125); coefficient or Dirichlet changes still recompute them, at the dt
+ * of that step. */
+int bfa_set_dt_caching(bfa_env *env, int keep_first);
+
 /* Internalised-substrate tracking, what Basic_Agent::simulate_secretion_and_uptake
  * does when default_microenvironment_options.
  * track_internalized_substrates_in_each_agent is set

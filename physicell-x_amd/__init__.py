@@ -72,6 +72,7 @@ def lib():
     L.bfa_track_internalized.argtypes = [_vp, ctypes.c_int, _dp, _dp]
     L.bfa_get_internalized.argtypes = [_vp, _dp]
     L.bfa_get_solver.argtypes = [_vp]
+    L.bfa_set_dt_caching.argtypes = [_vp, ctypes.c_int]
     L.bfa_upload.argtypes = [_vp, _dp, ctypes.c_int]
     L.bfa_download.argtypes = [_vp, _dp, ctypes.c_int]
     L.bfa_fill.argtypes = [_vp, _dp]
@@ -260,6 +261,11 @@ class Microenvironment:
         if name not in self.SOLVERS:
             raise BioFVMError(f"unknown solver {name!r}")
         self._check(lib().bfa_set_solver(self._h, self.SOLVERS[name]), "set_solver")
+
+    def set_dt_caching(self, keep_first):
+        """keep_first=True keeps the Thomas coefficients of the first dt for
+        later dts, as the reference's solver does (diffusion_solver_setup_done)."""
+        self._check(lib().bfa_set_dt_caching(self._h, 1 if keep_first else 0), "set_dt_caching")
 
     def solver(self):
         v = lib().bfa_get_solver(self._h)

@@ -47,6 +47,7 @@ struct bfa_env {
     int max_lds = 65536;
     int cus = 256;  // compute units of the device
     int xw_occ = 1;  // resident blocks per CU of the chosen wave x-sweep kernel
+    int xw_occ_fr = 1;  // ... of its form that handles face rows itself (slab schedule)
     int xw_cap = 0;  // blocks per CU the wave x sweep launches (0: xw_occ)
     double* bulk_st = nullptr;   // bulk sources: T*S per value (device order), FP64
     double* bulk_ups = nullptr;  // bulk sources: U + S per value
@@ -110,6 +111,7 @@ struct bfa_env {
     int R = 0;
     int TX[3] = {0, 64, 64};
     int variant = BFA_LOD_3D_BLOCKING;  // which reference solver's Dirichlet passes
+    bool keep_first_dt = false;  // reference caching: coefficients of the first dt kept for later dts
     int slab = 0;  // k-planes per x/y slab pair (0: whole-volume sweeps)
     int split = 1;  // substrate groups swept on parallel streams (1 or 2)
 
@@ -178,7 +180,8 @@ int pick_L(int n, int pmax) {
 
 size_t lds_col(const bfa_env* e, int n, int L, int TX) {
     const int P = (n + L - 1) / L;
-    return (size_t)(2 * P * L + TX * P + P) * e->tsize;
+    const size_t stash = e->prec == BFA_FP32 ? (size_t)TX * P * L : 0;  // the FP32 form's copy of the values
+    return ((size_t)(2 * P * L + TX * P + P) + stash) * e->tsize;
 }
 
 size_t lds_x(const bfa_env* e, int R, int L) {
@@ -311,8 +314,10 @@ bfa::SweepArgs<T> sweep_args(const bfa_env* e, int axis) {
     for (int d = 0; d < e->ns; ++d)
         if (e->active[d]) act |= 1ull << d;
     // the z sweep's only Dirichlet pass is the one after it, which
-    // LOD_3D_AVX256D does not make
-    a.active = (axis == 2 && e->variant == BFA_LOD_3D_AVX256D) ? 0ull : act;
+    // LOD_3D_AVX256D does not make; with per-voxel planes the sweeps apply no
+    // Dirichlet values and standalone face passes run between them
+    // (enqueue_diffusion)
+    a.active = ((axis == 2 && e->variant == BFA_LOD_3D_AVX256D) || a.plane) ? 0ull : act;
     unsigned reg = 0;
     for (int f = 0; f < 6; ++f)
         if (e->reg[f]) reg |= 1u << f;
@@ -355,7 +360,12 @@ int launch_xw_L(bfa_env* e, hipStream_t st, const Win& w) {
     const long long groups = (w.count + RW - 1) / RW;
     // one resident wave of blocks fills the device; each wavefront then
     // strides over its row groups with the next one's loads in flight
-    const int occ = e->xw_cap > 0 ? std::min(e->xw_cap, e->xw_occ) : e->xw_occ;
+    // The slab schedule (x and y sweeps of a slab back to back) keeps the
+    // rows on y and z faces in the kernel; otherwise a face pass over the
+    // window's substrates follows the sweep and completes its Dirichlet pass.
+    const bool fr = e->slab > 0;
+    const int resident = fr ? e->xw_occ_fr : e->xw_occ;
+    const int occ = e->xw_cap > 0 ? std::min(e->xw_cap, resident) : resident;
     const long long want = std::max<long long>(1, (long long)e->cus * occ / w.nd);
     const unsigned bx = (unsigned)std::max<long long>(1, std::min<long long>((groups + 3) / 4, want));
     bfa::SweepArgs<T> a = sweep_args<T>(e, 0);
@@ -363,8 +373,22 @@ int launch_xw_L(bfa_env* e, hipStream_t st, const Win& w) {
     a.first = w.first;
     a.count = w.count;
     dim3 grid(bx, (unsigned)w.nd);
-    hipLaunchKernelGGL((bfa::k_sweep_x_wave<T, L, DEV>), grid, dim3(256), 0, st, a);
+    if (fr)
+        hipLaunchKernelGGL((bfa::k_sweep_x_wave<T, L, DEV, true>), grid, dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL((bfa::k_sweep_x_wave<T, L, DEV, false>), grid, dim3(256), 0, st, a);
     LAUNCH_CHECK(e, "x sweep (wave)");
+    if (!fr && a.active) {
+        unsigned long long group = 0;
+        for (int d = w.d0; d < w.d0 + w.nd; ++d) group |= 1ull << d;
+        a.active &= group;
+        if (a.active) {
+            const long long work = 2ll * e->nx * e->nz + 2ll * e->nx * e->ny;  // y and z faces
+            hipLaunchKernelGGL(bfa::k_dirichlet_faces<T>, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, st, a,
+                               1);
+            LAUNCH_CHECK(e, "x sweep face pass");
+        }
+    }
     return BFA_OK;
 }
 
@@ -418,11 +442,16 @@ int raise_lds_limits(bfa_env* e) {
                                            (int)need[a]));
     if (e->R == 0) {
         const void* fw = nullptr;
-#define BFA_XW(Lv) \
-    if (e->L[0] == Lv) fw = (const void*)bfa::k_sweep_x_wave<T, Lv, DEV>;
+        const void* fwr = nullptr;
+#define BFA_XW(Lv)                                                            \
+    if (e->L[0] == Lv) {                                                      \
+        fw = (const void*)bfa::k_sweep_x_wave<T, Lv, DEV, false>;             \
+        fwr = (const void*)bfa::k_sweep_x_wave<T, Lv, DEV, true>;             \
+    }
         BFA_XW(4) BFA_XW(8) BFA_XW(16) BFA_XW(32)
 #undef BFA_XW
         e->xw_occ = fw ? occupancy_blocks(fw, 256) : 1;
+        e->xw_occ_fr = fwr ? occupancy_blocks(fwr, 256) : 1;
     }
     return BFA_OK;
 }
@@ -546,6 +575,14 @@ int launch_agents(bfa_env* e) {
     return BFA_OK;
 }
 
+// Substrates whose Dirichlet values are applied (dirichlet_activation_vector).
+unsigned long long face_active(const bfa_env* e) {
+    unsigned long long act = 0;
+    for (int d = 0; d < e->ns; ++d)
+        if (e->active[d]) act |= 1ull << d;
+    return act;
+}
+
 // Drops the captured step graph.  Launches of it may still be queued or
 // running (bfa_step only enqueues them), and its kernel arguments point into
 // buffers that the caller is about to replace, so the handle's stream (which
@@ -628,7 +665,7 @@ int run_precompute(bfa_env* e, double dt) {
 int ensure_precompute(bfa_env* e, double dt) {
     if (!e->coef_set) return fail(e, BFA_ERR_STATE, "bfa_set_coefficients has not been called");
     if (!(dt > 0.0) || !std::isfinite(dt)) return fail(e, BFA_ERR_ARG, "dt must be positive, got %g", dt);
-    if (e->pre_valid && e->pre_dt == dt) return BFA_OK;
+    if (e->pre_valid && (e->pre_dt == dt || e->keep_first_dt)) return BFA_OK;
     invalidate_graph(e);
     int rc = e->prec == BFA_FP32 ? run_precompute<float>(e, dt) : run_precompute<double>(e, dt);
     if (rc) return rc;
@@ -637,7 +674,36 @@ int ensure_precompute(bfa_env* e, double dt) {
     return BFA_OK;
 }
 
+// The standalone face pass (apply_dirichlet_boundaries_conditions) on the
+// handle's stream, with the table or the per-voxel planes.
+int launch_faces(bfa_env* e) {
+    const long long work = 2ll * e->ny * e->nz + 2ll * e->nx * e->nz + 2ll * e->nx * e->ny;
+    const unsigned blocks = (unsigned)((work + 255) / 256);
+    if (e->prec == BFA_FP32) {
+        bfa::SweepArgs<float> a = sweep_args<float>(e, 0);
+        a.active = face_active(e);
+        hipLaunchKernelGGL(bfa::k_dirichlet_faces<float>, dim3(blocks), dim3(256), 0, e->stream, a, 0);
+    } else {
+        bfa::SweepArgs<double> a = sweep_args<double>(e, 0);
+        a.active = face_active(e);
+        hipLaunchKernelGGL(bfa::k_dirichlet_faces<double>, dim3(blocks), dim3(256), 0, e->stream, a, 0);
+    }
+    LAUNCH_CHECK(e, "dirichlet faces");
+    return BFA_OK;
+}
+
 int enqueue_diffusion(bfa_env* e) {
+    if (!e->hplanes.empty()) {
+        // per-voxel Dirichlet values: the reference's separate face passes
+        // around plain sweeps (BioFVM/BioFVM_solvers.cpp:345,682,731,778)
+        const bool any = face_active(e) != 0;
+        for (int a = 0; a < 3; ++a) {
+            int rc = any ? launch_faces(e) : BFA_OK;
+            if (rc == BFA_OK) rc = launch_sweep_any(e, a);
+            if (rc) return rc;
+        }
+        return (any && e->variant == BFA_LOD_3D_BLOCKING) ? launch_faces(e) : BFA_OK;
+    }
     if (e->split > 1 && e->ns > 1 && e->slab == 0) return enqueue_split(e);
     if (e->slab > 0 && e->R == 0) {
         int rc = enqueue_xy_slabs(e);
@@ -1330,16 +1396,7 @@ int bfa_apply_dirichlet(bfa_env* e) {
     const double dt = e->pre_valid ? e->pre_dt : 1.0;
     int rc = ensure_precompute(e, dt);
     if (rc) return rc;
-    const long long work = 2ll * e->ny * e->nz + 2ll * e->nx * e->nz + 2ll * e->nx * e->ny;
-    const unsigned blocks = (unsigned)((work + 255) / 256);
-    if (e->prec == BFA_FP32)
-        hipLaunchKernelGGL(bfa::k_dirichlet_faces<float>, dim3(blocks), dim3(256), 0, e->stream,
-                           sweep_args<float>(e, 0));
-    else
-        hipLaunchKernelGGL(bfa::k_dirichlet_faces<double>, dim3(blocks), dim3(256), 0, e->stream,
-                           sweep_args<double>(e, 0));
-    LAUNCH_CHECK(e, "dirichlet faces");
-    return BFA_OK;
+    return launch_faces(e);
 }
 
 int bfa_total_mass(bfa_env* e, double* out) {
@@ -1602,6 +1659,12 @@ int bfa_set_solver(bfa_env* e, int variant) {
 }
 
 int bfa_get_solver(const bfa_env* e) { return e ? e->variant : -1; }
+
+int bfa_set_dt_caching(bfa_env* e, int keep_first) {
+    if (!e) return BFA_ERR_ARG;
+    e->keep_first_dt = keep_first != 0;
+    return BFA_OK;
+}
 
 int bfa_set_slab(bfa_env* e, int planes) {
     if (!e) return BFA_ERR_ARG;

@@ -149,29 +149,31 @@ __global__ void k_agents_f32(float* __restrict__ u, long long N, int ns, int nse
     *p = (float)((rho - b) * rs);
 }
 
-// ---- standalone Dirichlet pass over the six faces ---------------------------
-// Thread space: [x faces 2*ny*nz][y faces 2*nx*nz][z faces 2*nx*ny]; voxels
-// on edges are written more than once, always with the same value.
+// ---- standalone Dirichlet pass over the faces -------------------------------
+// Thread space: [y faces 2*nz*nx][z faces 2*ny*nx][x faces 2*nz*ny], x fastest
+// inside the y and z faces so that their writes are coalesced; yz_only skips
+// the x faces (after an x sweep, whose kernel writes the x-end nodes itself).
+// Voxels on edges are written more than once, always with the same value.
 template <typename T>
-__global__ void k_dirichlet_faces(SweepArgs<T> a) {
-    const long long nxf = 2ll * a.ny * a.nz, nyf = 2ll * a.nx * a.nz, nzf = 2ll * a.nx * a.ny;
+__global__ void k_dirichlet_faces(SweepArgs<T> a, int yz_only) {
+    const long long fy = (long long)a.nz * a.nx, fz = (long long)a.ny * a.nx, fx = (long long)a.nz * a.ny;
     long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int i, j, k;
-    if (g < nxf) {
-        i = (g & 1) ? a.nx - 1 : 0;
-        g >>= 1;
-        k = (int)(g % a.nz);
-        j = (int)(g / a.nz);
-    } else if ((g -= nxf) < nyf) {
-        j = (g & 1) ? a.ny - 1 : 0;
-        g >>= 1;
-        k = (int)(g % a.nz);
-        i = (int)(g / a.nz);
-    } else if ((g -= nyf) < nzf) {
-        k = (g & 1) ? a.nz - 1 : 0;
-        g >>= 1;
-        j = (int)(g % a.ny);
-        i = (int)(g / a.ny);
+    if (g < 2 * fy) {
+        j = g >= fy ? a.ny - 1 : 0;
+        g -= g >= fy ? fy : 0;
+        k = (int)(g / a.nx);
+        i = (int)(g - (long long)k * a.nx);
+    } else if ((g -= 2 * fy) < 2 * fz) {
+        k = g >= fz ? a.nz - 1 : 0;
+        g -= g >= fz ? fz : 0;
+        j = (int)(g / a.nx);
+        i = (int)(g - (long long)j * a.nx);
+    } else if (!yz_only && (g -= 2 * fz) < 2 * fx) {
+        i = g >= fx ? a.nx - 1 : 0;
+        g -= g >= fx ? fx : 0;
+        k = (int)(g / a.ny);
+        j = (int)(g - (long long)k * a.ny);
     } else {
         return;
     }

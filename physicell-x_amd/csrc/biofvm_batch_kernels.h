@@ -121,7 +121,8 @@ __global__ __launch_bounds__(256) void kb_sweep_x(const EnvDesc<T>* __restrict__
 // (T >= 16 * PMAX).  The ticket of the next item is drawn while the current
 // one runs (two LDS slots, alternating), the coefficients are staged once per
 // item, and the loads of tile q+1 are issued before tile q is solved.
-// Dynamic LDS: (2*PMAX*L + T + PMAX) * sizeof(T) + 16.  Chunks of 128 bytes
+// Dynamic LDS: (2*PMAX*L + T + PMAX) * sizeof(T) + 16, plus T*L*sizeof(T) for
+// the FP32 form's copy of the values (kb_col_lds).  Chunks of 128 bytes
 // or more take blocks of at most kb_col_max_threads(L * sizeof(T)) = 512
 // threads, which leaves them the registers to stay out of scratch.
 __host__ __device__ constexpr int kb_col_max_threads(int chunk_bytes) {
@@ -140,6 +141,7 @@ __global__ __launch_bounds__(kb_col_max_threads(L * sizeof(T))) void kb_sweep_co
     T* s_z = s_cp + PMAX * L;
     T* s_g = s_z + blockDim.x;
     int* s_item = reinterpret_cast<int*>(s_g + PMAX);
+    T* s_v = reinterpret_cast<T*>(s_item + 4);  // FP32: blockDim.x * L values (stash of v)
     const int t = threadIdx.x;
     const int n_items = *nitems;
     if (t == 0) s_item[0] = atomicAdd(ticket, 1);
@@ -156,7 +158,7 @@ __global__ __launch_bounds__(kb_col_max_threads(L * sizeof(T))) void kb_sweep_co
         }
         const SweepArgs<T> a = args_of<T>(envs[w.env], AXIS);
         const int nx = a.nx, ny = a.ny, nz = a.nz;
-        const int n = AXIS == 1 ? ny : nz;
+        int n = AXIS == 1 ? ny : nz;
         const int P = (n + L - 1) / L;
         const int W = col_tile_width(blockDim.x, P);
         const int lane = t % W;
@@ -167,7 +169,7 @@ __global__ __launch_bounds__(kb_col_max_threads(L * sizeof(T))) void kb_sweep_co
         const int ncols = AXIS == 1 ? nz * nx : ny * nx;
         const size_t stride = AXIS == 1 ? (size_t)nx : (size_t)nx * ny;
         T* ub = a.u + (long long)d * a.N;
-        const int s = p * L;
+        int s = p * L;
         const T bt = DEV ? a.beta[d] : T(0);
         const T kd = a.k[d];
         const T lam = DEV ? a.k[a.ns + d] : T(0);
@@ -197,6 +199,11 @@ __global__ __launch_bounds__(kb_col_max_threads(L * sizeof(T))) void kb_sweep_co
         fetch(act_n, base_n);
         if (t == 0) s_item[slot ^ 1] = nxt;
         for (int q = 0; q < w.ntiles; ++q) {
+            // the chunk position and line length are invariant over the
+            // item's tiles; laundering them per tile keeps the position tests
+            // (a scalar mask register pair each) from being hoisted out of
+            // the tile loop and spilling the scalar registers
+            asm volatile("" : "+v"(s), "+v"(n));
             T x[L];
 #pragma unroll
             for (int m = 0; m < L; ++m) x[m] = xn[m];
@@ -209,12 +216,12 @@ __global__ __launch_bounds__(kb_col_max_threads(L * sizeof(T))) void kb_sweep_co
                 fetch(act_n, base_n);
             }
             if (DEV) {
-                T r[L];
-                delta_residual<T, L>(x, hl, hr, s, act ? n : 0, kd, lam, bt, r);
+                // the values wait in LDS for x = v + delta (each thread its own entries)
+                col_stash<T, L>(s_v, x, p, lane, W);
+                delta_residual<T, L>(x, hl, hr, s, act ? n : 0, kd, lam, bt);
                 __syncthreads();  // staging done / the previous tile's fold readers are done
-                chunk_solve<T, L>(r, act, p, P, lane, W, s_inv, s_cp, kd, s_z, s_g);
-#pragma unroll
-                for (int m = 0; m < L; ++m) x[m] = x[m] + r[m];
+                chunk_solve<T, L>(x, act, p, P, lane, W, s_inv, s_cp, kd, s_z, s_g);
+                col_unstash_add<T, L>(s_v, x, p, lane, W);
             } else {
                 __syncthreads();  // staging done / the previous tile's fold readers are done
                 chunk_solve<T, L>(x, act, p, P, lane, W, s_inv, s_cp, kd, s_z, s_g);
@@ -353,7 +360,7 @@ __global__ __launch_bounds__(XYShape<T>::threads) void kb_sweep_xy(
 #pragma unroll
                 for (int m = 0; m < LX; ++m) q.v[m] = (have && m < xvalid) ? plane[j * nx + sx + m] : T(0);
                 x_row_dir_fetch<T, LX>(ax, d, k * ny + j, have, cx, q);
-                x_row_compute<T, LX, DEV>(nx, have, px, PX, sx, cx, q);
+                x_row_compute<T, LX, DEV>(nx, have, px, PX, sx, xvalid, cx, q, plane + j * nx + sx);
                 if (have) {
 #pragma unroll
                     for (int m = 0; m < LX; ++m)
@@ -368,16 +375,18 @@ __global__ __launch_bounds__(XYShape<T>::threads) void kb_sweep_xy(
                 T x[LY];
                 const T* col = plane + sy * nx + i;
                 if (DEV) {
+                    // the plane keeps v for x = v + delta: the column's own
+                    // nodes are rewritten only by this thread, after the read
 #pragma unroll
                     for (int m = 0; m < LY; ++m) x[m] = (act && sy + m < ny) ? col[m * nx] : T(0);
                     const T hl = (act && sy > 0) ? col[-nx] : T(0);
                     const T hr = (act && sy + LY < ny) ? col[LY * nx] : T(0);
-                    T r[LY];
-                    delta_residual<T, LY>(x, hl, hr, sy, act ? ny : 0, ky, lam, bt, r);
+                    delta_residual<T, LY>(x, hl, hr, sy, act ? ny : 0, ky, lam, bt);
                     __syncthreads();  // the previous tile's fold readers are done
-                    chunk_solve<T, LY>(r, act, py, PY, ly, W, s_inv, s_cp, ky, s_z, s_g);
+                    chunk_solve<T, LY>(x, act, py, PY, ly, W, s_inv, s_cp, ky, s_z, s_g);
 #pragma unroll
-                    for (int m = 0; m < LY; ++m) x[m] = x[m] + r[m];
+                    for (int m = 0; m < LY; ++m)
+                        if (act && sy + m < ny) x[m] = col[m * nx] + x[m];
                 } else {
 #pragma unroll
                     for (int m = 0; m < LY; ++m) x[m] = (act && sy + m < ny) ? col[m * nx] : T(0);

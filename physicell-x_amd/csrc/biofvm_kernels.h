@@ -57,7 +57,10 @@ struct SweepArgs {
     const T* beta;    // [ns] deviation shift (FP32 form), unused for FP64
     const T* dval;    // [7][ns] Dirichlet value per face slot
     // Per-voxel Dirichlet values of the six face planes, or nullptr to use
-    // dval.  Substrate d's planes start at plane + d*shell, in the order
+    // dval; read only by the standalone face pass (k_dirichlet_faces): with
+    // planes the sweeps apply no Dirichlet values themselves and the face
+    // pass runs between them.  Substrate d's planes start at plane + d*shell,
+    // in the order
     //   xmin [k][j], xmax [k][j]   (ny*nz each)
     //   ymin [k][i], ymax [k][i]   (nx*nz each)
     //   zmin [j][i], zmax [j][i]   (nx*ny each)
@@ -220,21 +223,38 @@ __device__ __forceinline__ void stage_coefs(T* s_inv, T* s_cp, const T* __restri
 // set by k, lam and beta alone, and the coefficient rounding only changes how
 // fast delta converges.
 // x[] holds the chunk's values at nodes s..s+L-1 of a line of n nodes, vl and
-// vr those at s-1 and s+L (ignored past the line ends); r[] gets the residual,
-// zero past the line end.
+// vr those at s-1 and s+L (ignored past the line ends); x[] is overwritten
+// with the residual, zero past the line end.  The values themselves are kept
+// by the caller (in LDS) for x = v + delta after the solve.
+// Each link difference v[i+1] - v[i] is formed once and serves both nodes
+// (lap_i = d_i - d_{i-1}, with d = 0 across a line end).
 template <typename T, int L>
-__device__ __forceinline__ void delta_residual(const T (&x)[L], T vl, T vr, int s, int n, T k, T lam,
-                                               T beta, T (&r)[L]) {
+__device__ __forceinline__ void delta_residual(T (&x)[L], T vl, T vr, int s, int n, T k, T lam, T beta) {
+    T dprev = s > 0 ? x[0] - vl : T(0);
 #pragma unroll
     for (int m = 0; m < L; ++m) {
         const int pos = s + m;
-        const T left = m == 0 ? vl : x[m - 1];
+        const T v = x[m];
         const T right = m == L - 1 ? vr : x[m + 1];
-        T lap = T(0);
-        if (pos > 0) lap = left - x[m];
-        if (pos < n - 1) lap = lap + (right - x[m]);
-        r[m] = pos < n ? fma(k, lap, fma(-lam, x[m], -beta)) : T(0);
+        const T dnext = pos < n - 1 ? right - v : T(0);
+        x[m] = pos < n ? fma(k, dnext - dprev, fma(-lam, v, -beta)) : T(0);
+        dprev = dnext;
     }
+}
+
+// A column chunk's values in LDS, node-major with the TX columns of a block
+// side by side (conflict-free): node m of chunk p, column lane at
+// s_v[(p*L + m)*TX + lane].
+template <typename T, int L>
+__device__ __forceinline__ void col_stash(T* s_v, const T (&x)[L], int p, int lane, int TX) {
+#pragma unroll
+    for (int m = 0; m < L; ++m) s_v[(p * L + m) * TX + lane] = x[m];
+}
+
+template <typename T, int L>
+__device__ __forceinline__ void col_unstash_add(const T* s_v, T (&x)[L], int p, int lane, int TX) {
+#pragma unroll
+    for (int m = 0; m < L; ++m) x[m] = s_v[(p * L + m) * TX + lane] + x[m];
 }
 
 // ---- x sweep: R rows per block staged through LDS ---------------------------
@@ -291,11 +311,12 @@ __global__ __launch_bounds__(1024) void k_sweep_x(SweepArgs<T> a, int R, int log
         for (int m = 0; m < L; ++m) {
             const int pos = s + m;
             if (pos < nx && (rowb || pos == 0 || pos == nx - 1))
-                x[m] = dir_value(a, d, pos, jj, kk);
+                x[m] = a.dval[dir_slot(pos, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
         }
     }
     if (DEV) {
-        // the neighbours of the chunk's end nodes, after the Dirichlet pass
+        // the neighbours of the chunk's end nodes, after the Dirichlet pass;
+        // the tile keeps v for x = v + delta
         if (act) {
 #pragma unroll
             for (int m = 0; m < L; ++m)
@@ -304,11 +325,13 @@ __global__ __launch_bounds__(1024) void k_sweep_x(SweepArgs<T> a, int R, int log
         __syncthreads();
         const T vl = (act && s > 0) ? tile[rho * S + s - 1] : T(0);
         const T vr = (act && s + L < nx) ? tile[rho * S + s + L] : T(0);
-        T r[L];
-        delta_residual<T, L>(x, vl, vr, s, act ? nx : 0, a.k[d], a.k[a.ns + d], a.beta[d], r);
-        chunk_solve<T, L>(r, act, p, P, rho, R, s_inv, s_cp, a.k[d], s_z, s_g);
+        delta_residual<T, L>(x, vl, vr, s, act ? nx : 0, a.k[d], a.k[a.ns + d], a.beta[d]);
+        chunk_solve<T, L>(x, act, p, P, rho, R, s_inv, s_cp, a.k[d], s_z, s_g);
+        if (act) {
 #pragma unroll
-        for (int m = 0; m < L; ++m) x[m] = x[m] + r[m];
+            for (int m = 0; m < L; ++m)
+                if (s + m < nx) x[m] = tile[rho * S + s + m] + x[m];
+        }
     } else {
         chunk_solve<T, L>(x, act, p, P, rho, R, s_inv, s_cp, a.k[d], s_z, s_g);
     }
@@ -317,7 +340,7 @@ __global__ __launch_bounds__(1024) void k_sweep_x(SweepArgs<T> a, int R, int log
         for (int m = 0; m < L; ++m) {
             const int pos = s + m;
             if (pos < nx && (rowb || pos == 0 || pos == nx - 1))
-                x[m] = dir_value(a, d, pos, jj, kk);
+                x[m] = a.dval[dir_slot(pos, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
         }
     }
     if (act) {
@@ -450,7 +473,6 @@ struct XCoef {
     T k, bt, lam;     // D*dt/dx^2, the deviation shift and decay*dt/3
     T v_lo, v_hi;     // Dirichlet values at the x ends of rows on no y/z face
     bool dact;        // Dirichlet on for this substrate
-    bool plane;       // per-voxel Dirichlet planes (values fetched per row)
 };
 
 template <typename T, int L, bool DEV>
@@ -466,7 +488,6 @@ __device__ __forceinline__ void x_coef_setup(const SweepArgs<T>& a, int d, int p
     c.bt = DEV ? a.beta[d] : T(0);
     c.lam = DEV ? a.k[a.ns + d] : T(0);
     c.dact = (a.active >> d) & 1ull;
-    c.plane = a.plane != nullptr;
     c.v_lo = a.dval[dir_slot_xonly(0, nx, a.reg) * a.ns + d];
     c.v_hi = a.dval[dir_slot_xonly(nx - 1, nx, a.reg) * a.ns + d];
     T za = T(1);
@@ -501,37 +522,27 @@ __device__ __forceinline__ void x_coef_setup(const SweepArgs<T>& a, int d, int p
 template <typename T, int L>
 struct XRow {
     T v[L];
-    T f_lo, f_hi, f_in;  // Dirichlet values: the row's x ends, and the interior of a row on a y or z face
+    T f_lo, f_hi, f_in;  // Dirichlet values of a row on a y or z face
     bool rowb;           // the row is on a y or z face and Dirichlet is on
-    const T* fp;         // per-voxel planes: the y or z plane row holding this face row's values
 };
 
 // The per-row Dirichlet part of x_row_fetch: rowb and the face values of
-// row r (= k*ny + j) when it lies on a y or z face.
-template <typename T, int L>
+// row r (= k*ny + j) when it lies on a y or z face.  FR = false leaves rows on
+// y and z faces to a face pass after the sweep (k_dirichlet_faces): every row
+// then takes only its two x-end values, from registers, which keeps the
+// per-row face lookups out of the kernel (rows are solved independently, so
+// a face row's result only has to be overwritten before the next sweep).
+template <typename T, int L, bool FR = true>
 __device__ __forceinline__ void x_row_dir_fetch(const SweepArgs<T>& a, int d, int r, bool have,
                                                 const XCoef<T, L>& c, XRow<T, L>& q) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     q.f_lo = q.f_hi = q.f_in = T(0);
     q.rowb = false;
-    q.fp = nullptr;
-    if (have && c.dact) {
+    if (FR && have && c.dact) {
         const int kk = r / ny;
         const int jj = r - kk * ny;
         q.rowb = (jj == 0 || jj == ny - 1 || kk == 0 || kk == nz - 1);
-        if (a.plane) {
-            // per-voxel values: the row's two x-end values (x planes are
-            // indexed by r = k*ny + j), and for a face row its y or z plane row
-            const T* pl = a.plane + (long long)d * a.shell;
-            const PlaneOffsets o = plane_offsets(a);
-            q.f_lo = pl[o.xlo + r];
-            q.f_hi = pl[o.xhi + r];
-            if (q.rowb)
-                q.fp = pl + (jj == 0        ? o.ylo + (long long)kk * nx
-                             : jj == ny - 1 ? o.yhi + (long long)kk * nx
-                             : kk == 0      ? o.zlo + (long long)jj * nx
-                                            : o.zhi + (long long)jj * nx);
-        } else if (q.rowb) {
+        if (q.rowb) {
             q.f_lo = a.dval[dir_slot(0, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
             q.f_hi = a.dval[dir_slot(nx - 1, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
             q.f_in = a.dval[dir_slot(-1, jj, kk, nx, ny, nz, a.reg) * a.ns + d];
@@ -539,7 +550,7 @@ __device__ __forceinline__ void x_row_dir_fetch(const SweepArgs<T>& a, int d, in
     }
 }
 
-template <typename T, int L>
+template <typename T, int L, bool FR = true>
 __device__ __forceinline__ void x_row_fetch(const SweepArgs<T>& a, int d, int r, bool have, int s,
                                             int valid, const XCoef<T, L>& c, XRow<T, L>& q) {
     const T* ub = a.u + (long long)d * a.N;
@@ -549,23 +560,14 @@ __device__ __forceinline__ void x_row_fetch(const SweepArgs<T>& a, int d, int r,
 #pragma unroll
         for (int m = 0; m < L; ++m) q.v[m] = T(0);
     }
-    x_row_dir_fetch<T, L>(a, d, r, have, c, q);
+    x_row_dir_fetch<T, L, FR>(a, d, r, have, c, q);
 }
 
 // Dirichlet pass of one row chunk (see x_row_fetch).
 template <typename T, int L>
 __device__ __forceinline__ void x_row_dirichlet(T (&cur)[L], int nx, int s, bool have,
                                                 const XCoef<T, L>& c, const XRow<T, L>& q) {
-    if (q.fp) {  // face row with per-voxel values: every node from its plane
-#pragma unroll
-        for (int m = 0; m < L; ++m)
-            if (s + m < nx) cur[m] = q.fp[s + m];
-    } else if (c.plane && have && c.dact) {  // per-voxel values at the row's x ends
-        if (s == 0) cur[0] = q.f_lo;
-#pragma unroll
-        for (int m = 0; m < L; ++m)
-            if (s + m == nx - 1) cur[m] = q.f_hi;
-    } else if (q.rowb) {
+    if (q.rowb) {
         dirichlet_row<T, L>(cur, s, nx, true, q.f_lo, q.f_hi, q.f_in);
     } else if (have && c.dact) {
         if (s == 0) cur[0] = c.v_lo;
@@ -580,33 +582,34 @@ __device__ __forceinline__ void x_row_dirichlet(T (&cur)[L], int nx, int s, bool
 }
 
 // Dirichlet D0, the row solve and Dirichlet D1 on the chunk held in q.v.
+// FP64 solves in place on the values.  FP32 (DEV) solves for the increment
+// from the residual (delta_residual): the neighbours of the chunk's end nodes
+// are the last node of lane p-1 and the first of lane p+1 (lanes of other
+// rows are masked by the line-end tests), and after the solve the values are
+// read back from vs, this lane's chunk in LDS (valid nodes), for v + delta;
+// vs holds v at every node that is not a Dirichlet node (D1 rewrites those).
 template <typename T, int L, bool DEV>
-__device__ __forceinline__ void x_row_compute(int nx, bool have, int p, int P, int s,
-                                              const XCoef<T, L>& c, XRow<T, L>& q) {
+__device__ __forceinline__ void x_row_compute(int nx, bool have, int p, int P, int s, int valid,
+                                              const XCoef<T, L>& c, XRow<T, L>& q, const T* vs) {
     x_row_dirichlet<T, L>(q.v, nx, s, have, c, q);
-    // FP64: solve in place on the values.  FP32 (DEV): solve for the
-    // increment from the residual (delta_residual); the neighbours of the
-    // chunk's end nodes are the last node of lane p-1 and the first of lane
-    // p+1 (lanes of other rows are masked by the line-end tests).
-    T r[L];
     if (DEV) {
         const T vl = shfl_up_t(q.v[L - 1], 1);
         const T vr = shfl_down_t(q.v[0], 1);
-        delta_residual<T, L>(q.v, vl, vr, s, nx, c.k, c.lam, c.bt, r);
+        delta_residual<T, L>(q.v, vl, vr, s, nx, c.k, c.lam, c.bt);
     }
-    T(&cur)[L] = DEV ? r : q.v;
+    T(&cur)[L] = q.v;
     // past the row end 1/den = c' = 0, so whatever sits there leaves the
     // row's values unchanged and becomes 0 in the forward pass
     T zb = T(0);
 #pragma unroll
     for (int m = 0; m < L; ++m) zb = fma(c.k, zb, cur[m]) * c.inv[m];
+    // scan steps that are off have multiplier 0 (and a shuffle source inside
+    // the wave), so all six run unconditionally
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
         const int off = 1 << j;
-        if (off < P) {
-            const T pb = shfl_up_t(zb, off);
-            zb = fma(c.fm[j], pb, zb);
-        }
+        const T pb = shfl_up_t(zb, off);
+        zb = fma(c.fm[j], pb, zb);
     }
     T y = shfl_up_t(zb, 1);
     if (p == 0) y = T(0);
@@ -621,10 +624,8 @@ __device__ __forceinline__ void x_row_compute(int nx, bool have, int p, int P, i
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
         const int off = 1 << j;
-        if (off < P) {
-            const T nb = shfl_down_t(wb, off);
-            wb = fma(c.bm[j], nb, wb);
-        }
+        const T nb = shfl_down_t(wb, off);
+        wb = fma(c.bm[j], nb, wb);
     }
     T xn = shfl_down_t(wb, 1);
     if (p == P - 1) xn = T(0);
@@ -633,9 +634,22 @@ __device__ __forceinline__ void x_row_compute(int nx, bool have, int p, int P, i
         xn = fma(-c.cp[m], xn, cur[m]);
         cur[m] = xn;
     }
-    if (DEV) {
+    if (DEV && have) {
+        using VA = typename Vec16A<T>::type;
+        constexpr int W = Vec16<T>::n;
+        if (L % W == 0 && nx % W == 0) {  // vs is vector aligned and valid a multiple of W
 #pragma unroll
-        for (int m = 0; m < L; ++m) q.v[m] = q.v[m] + r[m];
+            for (int v = 0; v < L / W; ++v)
+                if (v * W < valid) {
+                    const VA t = *reinterpret_cast<const VA*>(vs + v * W);
+#pragma unroll
+                    for (int w = 0; w < W; ++w) q.v[v * W + w] = t[w] + q.v[v * W + w];
+                }
+        } else {
+#pragma unroll
+            for (int m = 0; m < L; ++m)
+                if (m < valid) q.v[m] = vs[m] + q.v[m];
+        }
     }
     if (have) x_row_dirichlet<T, L>(q.v, nx, s, have, c, q);
 }
@@ -657,7 +671,29 @@ __device__ __forceinline__ void x_row_solve_staged(const SweepArgs<T>& a, int d,
     using VU = typename Vec16<T>::type;
     constexpr int W = Vec16<T>::n;
     const int nx = a.nx;
-    x_row_compute<T, L, DEV>(nx, have, p, P, s, c, q);
+    // the lane's chunk position and count are loop-invariant; laundering them
+    // per row group keeps the compiler from hoisting every position test (a
+    // scalar mask register pair each) out of the row loop, which otherwise
+    // exhausts the scalar registers and spills them into vector lanes
+    asm volatile("" : "+v"(s), "+v"(valid));
+    if (DEV && have) {  // the chunk's values, read back after the delta solve
+        T* dst = st + rw * nx + s;
+        if (nx % W == 0) {
+#pragma unroll
+            for (int v = 0; v < L / W; ++v)
+                if (v * W < valid) {
+                    VA t;
+#pragma unroll
+                    for (int w = 0; w < W; ++w) t[w] = q.v[v * W + w];
+                    *reinterpret_cast<VA*>(dst + v * W) = t;
+                }
+        } else {
+#pragma unroll
+            for (int m = 0; m < L; ++m)
+                if (m < valid) dst[m] = q.v[m];
+        }
+    }
+    x_row_compute<T, L, DEV>(nx, have, p, P, s, valid, c, q, st + rw * nx + s);
     if (have) {
         T* dst = st + rw * nx + s;
         if (nx % W == 0) {  // chunk starts on a vector boundary; valid is a multiple of W
@@ -719,7 +755,7 @@ __device__ __forceinline__ void x_rows_pipelined(const SweepArgs<T>& a, int d, i
     }
 }
 
-template <typename T, int L, bool DEV>
+template <typename T, int L, bool DEV, bool FR>
 __global__ __launch_bounds__(256) void k_sweep_x_wave(SweepArgs<T> a) {
     const int nx = a.nx;
     const int P = (nx + L - 1) / L;
@@ -749,7 +785,7 @@ __global__ __launch_bounds__(256) void k_sweep_x_wave(SweepArgs<T> a) {
     XRow<T, L> nxt;
     {
         const int r = row0 + g * RW + rw;
-        x_row_fetch<T, L>(a, d, r, g < ngroups && lane_on && r < rows, s, valid, c, nxt);
+        x_row_fetch<T, L, FR>(a, d, r, g < ngroups && lane_on && r < rows, s, valid, c, nxt);
     }
     for (; g < ngroups; g += wstride) {
         XRow<T, L> cur = nxt;
@@ -757,7 +793,7 @@ __global__ __launch_bounds__(256) void k_sweep_x_wave(SweepArgs<T> a) {
         const int gn = g + wstride;
         if (gn < ngroups) {
             const int rn = row0 + gn * RW + rw;
-            x_row_fetch<T, L>(a, d, rn, lane_on && rn < rows, s, valid, c, nxt);
+            x_row_fetch<T, L, FR>(a, d, rn, lane_on && rn < rows, s, valid, c, nxt);
         }
         const int rg = row0 + g * RW;
         const int nrv = rows - rg < RW ? rows - rg : RW;
@@ -778,48 +814,6 @@ __device__ __forceinline__ void col_dirichlet(T (&x)[L], const SweepArgs<T>& a, 
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const bool lineb = AXIS == 1 ? (ci == 0 || ci == nx - 1 || co == 0 || co == nz - 1)
                                  : (ci == 0 || ci == nx - 1 || co == 0 || co == ny - 1);
-    if (a.plane) {
-        // per-voxel values.  Line ends: the y (z) planes at (k, i) ((j, i)).
-        // A line on a face parallel to it reads every node from that face's
-        // plane: x faces are indexed k*ny + j, y faces k*nx + i, z faces
-        // j*nx + i, so along the line the index steps by `ls`.
-        const T* pl = a.plane + (long long)d * a.shell;
-        const PlaneOffsets o = plane_offsets(a);
-        if (lineb) {
-            long long b0, ls;
-            if (AXIS == 1) {  // column (i = ci, k = co), nodes j
-                if (ci == 0 || ci == nx - 1) {
-                    b0 = (ci == 0 ? o.xlo : o.xhi) + (long long)co * ny;
-                    ls = 1;
-                } else {
-                    b0 = (co == 0 ? o.zlo : o.zhi) + ci;
-                    ls = nx;
-                }
-            } else {  // column (i = ci, j = co), nodes k
-                if (ci == 0 || ci == nx - 1) {
-                    b0 = (ci == 0 ? o.xlo : o.xhi) + co;
-                    ls = ny;
-                } else {
-                    b0 = (co == 0 ? o.ylo : o.yhi) + ci;
-                    ls = nx;
-                }
-            }
-#pragma unroll
-            for (int m = 0; m < L; ++m)
-                if (s + m < n) x[m] = pl[b0 + (long long)(s + m) * ls];
-        } else {
-            const long long e0 = AXIS == 1 ? o.ylo + (long long)co * nx + ci : o.zlo + (long long)co * nx + ci;
-            const long long e1 = AXIS == 1 ? o.yhi + (long long)co * nx + ci : o.zhi + (long long)co * nx + ci;
-            if (p == 0) x[0] = pl[e0];
-            if (p == P - 1) {
-                const T v = pl[e1];
-#pragma unroll
-                for (int m = 0; m < L; ++m)
-                    if (s + m == n - 1) x[m] = v;
-            }
-        }
-        return;
-    }
     const int s_lo = AXIS == 1 ? dir_slot(ci, 0, co, nx, ny, nz, a.reg)
                                : dir_slot(ci, co, 0, nx, ny, nz, a.reg);
     const int s_hi = AXIS == 1 ? dir_slot(ci, n - 1, co, nx, ny, nz, a.reg)
@@ -866,7 +860,8 @@ __device__ __forceinline__ void store_line_chunk(const T (&x)[L], T* line, size_
 }
 
 // blockDim.x = TX*P with P = ceil(n/L); grid = (ceil(ncols/TX), ns).
-// Dynamic LDS: (2*P*L + TX*P + P) * sizeof(T).
+// Dynamic LDS: (2*P*L + TX*P + P) * sizeof(T), plus TX*P*L * sizeof(T) for the
+// FP32 form's copy of the values.
 template <typename T, int L, int AXIS, bool DEV, int TX>
 __global__ __launch_bounds__(1024) void k_sweep_col(SweepArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -906,15 +901,16 @@ __global__ __launch_bounds__(1024) void k_sweep_col(SweepArgs<T> a) {
     load_line_chunk<T, L>(x, line, stride, cnt);
     if (DEV) {
         // delta form: the chunk's outer neighbours come from the field (the
-        // neighbouring chunks' threads load them too, so they hit in cache)
+        // neighbouring chunks' threads load them too, so they hit in cache);
+        // the values wait in LDS for x = v + delta
         const T vl = (act && s > 0) ? *(line - stride) : T(0);
         const T vr = (act && s + L < n) ? line[(size_t)L * stride] : T(0);
-        T r[L];
-        delta_residual<T, L>(x, vl, vr, s, act ? n : 0, a.k[d], a.k[a.ns + d], a.beta[d], r);
+        T* s_v = s_g + P;
+        col_stash<T, L>(s_v, x, p, lane, TX);
+        delta_residual<T, L>(x, vl, vr, s, act ? n : 0, a.k[d], a.k[a.ns + d], a.beta[d]);
         __syncthreads();  // coefficient staging complete
-        chunk_solve<T, L>(r, act, p, P, lane, TX, s_inv, s_cp, a.k[d], s_z, s_g);
-#pragma unroll
-        for (int m = 0; m < L; ++m) x[m] = x[m] + r[m];
+        chunk_solve<T, L>(x, act, p, P, lane, TX, s_inv, s_cp, a.k[d], s_z, s_g);
+        col_unstash_add<T, L>(s_v, x, p, lane, TX);
     } else {
         __syncthreads();  // coefficient staging complete
         chunk_solve<T, L>(x, act, p, P, lane, TX, s_inv, s_cp, a.k[d], s_z, s_g);

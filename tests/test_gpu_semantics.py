@@ -227,3 +227,29 @@ def test_internalized_tracking_from_agent_state(pkg):
         assert relmax(env.download(), u) <= FP64_TOL
     finally:
         env.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("fp32", FP32_TOL)])
+def test_dt_caching_keeps_the_first_coefficients(pkg, precision, tol):
+    """With dt caching the coefficients of the first dt serve every later dt,
+    as the reference's solver keeps them (diffusion_solver_setup_done); by
+    default a new dt recomputes them."""
+    p = make_problem(20, 17, 23, 2, seed=71, dirichlet="all")
+    env = gpu_env(pkg, p, precision)
+    try:
+        env.set_dt_caching(True)
+        orc = oracle_for(p)
+        u = p.u0.copy()
+        for dt in (0.01, 0.02, 0.01):
+            orc.diffusion_step(u, 0.01)  # the first dt's coefficients every time
+            env.simulate_diffusion_decay(dt)
+        assert relmax(env.download(), u) <= tol
+        env.set_dt_caching(False)
+        env.upload(p.u0)
+        orc2 = oracle_for(p)
+        v = p.u0.copy()
+        orc2.diffusion_step(v, 0.02)
+        env.simulate_diffusion_decay(0.02)
+        assert relmax(env.download(), v) <= tol
+    finally:
+        env.close()
