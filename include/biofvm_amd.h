@@ -224,6 +224,18 @@ int bfa_gradients(bfa_env *env, double *out, int layout);
  * Microenvironment::nearest_gradient_vector (BioFVM_microenvironment.cpp:1443-1452)
  * by chemotaxis.  Only the listed voxels are computed. */
 int bfa_gradients_at(bfa_env *env, int n, const int32_t *voxel, double *out);
+/* bfa_gradients_at without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same arguments and
+ * results as bfa_gradients_at; its device staging (the voxel list and the
+ * gradients before the copy back) lives in a static device array of
+ * bfa_noalloc_scratch_bytes() bytes per GPU (64 MiB), held until the process
+ * ends, instead of per-call allocations: n * (4 + 24 * ns) bytes (the voxel
+ * list rounded up to 256 bytes) must fit it, else BFA_ERR_ARG.  Every call on
+ * one GPU shares the array, so calls that overlap on one GPU race; this
+ * library serialises the calls of one process. */
+int bfa_gradients_at_noalloc(bfa_env *env, int n, const int32_t *voxel, double *out);
+size_t bfa_noalloc_scratch_bytes(void);
 /* Mean device time (ms, HIP events) of the whole-field gradient kernel over
  * reps launches into a device scratch buffer (measurement; no reference
  * counterpart). */

@@ -117,6 +117,7 @@ struct bfa_env {
 
     hipStream_t aux[2] = {nullptr, nullptr};
     hipEvent_t ev_fork = nullptr;
+    void* scratch = nullptr;  // device address of the static scratch array on this device
     hipEvent_t ev_join[2] = {nullptr, nullptr};
 
     std::string err;
@@ -778,6 +779,8 @@ int reduce(bfa_env* e, const void* x, bool f32, long long count, long long strid
     return BFA_OK;
 }
 
+int scratch_setup(bfa_env* e);  // biofvm_noalloc.inc
+
 // A new agent list of another length has other agent identities: the
 // tracked totals (indexed by agent) no longer apply and tracking stops.
 void drop_tracking_if_resized(bfa_env* e, int n) {
@@ -885,6 +888,7 @@ int bfa_create(bfa_env** out, int nx, int ny, int nz, int ns, double dx, int pre
         (rc = dalloc(e, &e->red_out, (size_t)ns * 8)))
         return bail(rc);
     if ((rc = apply_tiles(e))) return bail(rc);
+    if ((rc = scratch_setup(e))) return bail(rc);
     st = hipStreamSynchronize(e->stream);
     if (st != hipSuccess) {
         fail(e, BFA_ERR_HIP, "initial sync failed: %s", hipGetErrorString(st));
@@ -1780,3 +1784,4 @@ int bfa_get_tiles(const bfa_env* e, int* out6) {
 // ---- batches of independent microenvironments ------------------------------
 #include "biofvm_batch_host.inc"
 #include "biofvm_agents_dev.inc"
+#include "biofvm_noalloc.inc"
