@@ -228,7 +228,7 @@ def ensemble_members(n_env, seed):
     return out
 
 
-def run_ensemble(pkg, n_env=256, steps=100, repeats=3, seed=4, sequential=True, tune=True):
+def run_ensemble(pkg, n_env=256, steps=100, repeats=20, seed=4, sequential=True, tune=True):
     """Batched ensemble vs sequential plain solves of the same members.  The
     batch kernel shapes are autotuned on the batch itself, after which every
     member is reset to its initial state."""
@@ -262,7 +262,10 @@ def run_ensemble(pkg, n_env=256, steps=100, repeats=3, seed=4, sequential=True, 
             b.synchronize()
             lat.append(time.perf_counter() - t0)
         lat = np.array(lat)
-        res[prec] = {"value": updates / float(np.median(lat)), "p50_s": float(np.percentile(lat, 50)),
+        res[prec] = {"value": updates / float(np.median(lat)),
+                     "alg_frac_of_peak": updates / float(np.median(lat)) * (24 if prec == "fp32" else 48)
+                     / (PEAK_HBM_GBS * 1e9),
+                     "p50_s": float(np.percentile(lat, 50)),
                      "p99_s": float(np.percentile(lat, 99)), "runs": repeats, "tiles": list(b.tiles()),
                      "col_threads": list(b.col_threads()), "streams": b.streams(),
                      "autotune_trials": tuned["trials"] if tuned else 0}
@@ -325,7 +328,7 @@ def serving_requests(n_req, seed, rate=200.0, edge=(48, 256), ns_max=8, steps=(1
     return reqs, offs
 
 
-def run_serving(pkg, n_req=40, seed=5, rate=200.0, budget=200e9, k=10, spot=2, tiles=None, sequential=True,
+def run_serving(pkg, n_req=200, seed=5, rate=200.0, budget=200e9, k=10, spot=2, tiles=None, sequential=True,
                 solo_voxels=SOLO_VOXELS):
     import importlib
     S = importlib.import_module("physicellx_amd.serving")
@@ -352,6 +355,8 @@ def run_serving(pkg, n_req=40, seed=5, rate=200.0, budget=200e9, k=10, spot=2, t
            "iterations": srv.iterations, "solo_voxels": solo_voxels,
            "solo_jobs": sum(1 for j in done if srv.is_solo(j))}
     res.update(S.latency_stats(done))
+    # sweep traffic of the served FP32 updates (24 B each) against the 8 TB/s peak
+    res["alg_frac_of_peak"] = res["value"] * 24 / (PEAK_HBM_GBS * 1e9)
     # FP64 spot check of the smallest requests, run alone
     errs = []
     for i in spot_ids:
@@ -528,7 +533,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--mode", default="default", choices=["default", "single", "ensemble", "serve", "large"])
-    ap.add_argument("--requests", type=int, default=40)
+    ap.add_argument("--requests", type=int, default=200)
     ap.add_argument("--no-seq", action="store_true", help="ensemble mode: skip the sequential plain solves")
     ap.add_argument("--tiles", default="", help="fixed FP32 shapes Lx,Ly,Lz,rows_x,cols_y,cols_z[,slab[,streams[,x_grid]]] "
                     "instead of autotuning (profiling runs)")
@@ -639,7 +644,7 @@ def main():
     out.update(res)
     if args.mode == "default":
         out["large"] = run_large(pkg)
-        ens = run_ensemble(pkg, repeats=3)
+        ens = run_ensemble(pkg, repeats=20)
         srv = run_serving(pkg, n_req=args.requests)
         out["batched_solve"] = {"p50_s": ens["fp32"]["p50_s"], "p99_s": ens["fp32"]["p99_s"],
                                 "workload": "256-member ensemble, 100 steps per solve, FP32"}
