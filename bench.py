@@ -137,15 +137,31 @@ def run_host_boundary(env, dt, updates, reps=3):
             "value": updates / (total * 1e-3), "field_bytes_each_way": int(host.nbytes), "reps": reps}
 
 
-def load_traffic(kernel_key):
+def load_traffic(kernel_key, tiles):
+    """HBM bytes per launch of a sweep kernel, from the counter runs recorded
+    in profiles/traffic.json, or None unless an entry was measured with this
+    exact kernel shape (tiles = Lx, Ly, Lz, rows_x, cols_y, cols_z): another
+    shape moves other bytes, so its figure is not reported."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     try:
-        doc = json.load(open(path))
-        return doc.get(kernel_key)
+        ent = json.load(open(path)).get("entries", {}).get(kernel_key)
     except Exception:
         return None
+    if not ent or [int(t) for t in ent.get("tiles", [])] != [int(t) for t in list(tiles)[:6]]:
+        return None
+    return ent.get("bytes")
+
+
+def parse_tiles(spec):
+    """Lx,Ly,Lz,rows_x,cols_y,cols_z[,slab[,streams[,x_grid]]] -> shape dict."""
+    if not spec:
+        return None
+    v = [int(x) for x in spec.split(",")]
+    return dict(Lx=v[0], Ly=v[1], Lz=v[2], rows_x=v[3], cols_y=v[4], cols_z=v[5],
+                slab=v[6] if len(v) > 6 else 0, substrate_streams=v[7] if len(v) > 7 else 1,
+                x_grid=v[8] if len(v) > 8 else 0)
 
 
 def host_cpu():
@@ -228,10 +244,11 @@ def ensemble_members(n_env, seed):
     return out
 
 
-def run_ensemble(pkg, n_env=256, steps=100, repeats=20, seed=4, sequential=True, tune=True):
+def run_ensemble(pkg, n_env=256, steps=100, repeats=20, seed=4, sequential=True, tune=True, shapes=None):
     """Batched ensemble vs sequential plain solves of the same members.  The
     batch kernel shapes are autotuned on the batch itself, after which every
-    member is reset to its initial state."""
+    member is reset to its initial state; shapes = {precision: (Lx, Ly, Lz,
+    cols_y, cols_z, col_threads, streams)} pins them instead (profiling runs)."""
     import importlib
     AT = importlib.import_module("physicellx_amd.autotune")
     mem = ensemble_members(n_env, seed)
@@ -248,7 +265,13 @@ def run_ensemble(pkg, n_env=256, steps=100, repeats=20, seed=4, sequential=True,
             b.set_agents(mid, m["voxel"], m["c1"], m["c2"], m["e2"])
             ids.append(mid)
         tuned = None
-        if tune:
+        shape = (shapes or {}).get(prec)
+        if shape is not None:
+            b.set_col_threads(0)
+            b.set_tiles(*shape[:5])
+            b.set_col_threads(shape[5])
+            b.set_streams(shape[6])
+        elif tune:
             tuned = AT.autotune_batch(b, 0.01)
             for mid, m in zip(ids, mem):
                 b.fill(mid, m["dval"])
@@ -268,7 +291,7 @@ def run_ensemble(pkg, n_env=256, steps=100, repeats=20, seed=4, sequential=True,
                      "p50_s": float(np.percentile(lat, 50)),
                      "p99_s": float(np.percentile(lat, 99)), "runs": repeats, "tiles": list(b.tiles()),
                      "col_threads": list(b.col_threads()), "streams": b.streams(),
-                     "autotune_trials": tuned["trials"] if tuned else 0}
+                     "autotune_trials": tuned["trials"] if tuned else 0, "pinned": shape is not None}
         b.close()
     errs = [float(np.max(np.abs(a - c)) / np.max(np.abs(c))) for a, c in zip(fields["fp32"], fields["fp64"])]
     res["fp32_vs_fp64_max_rel_err"] = max(errs)
@@ -459,11 +482,12 @@ def large_workload(n=800, seed=3, na=1_000_000):
                 S=S, U=U, E=E, rs=rs, V=V)
 
 
-def run_large(pkg, n=800, steps=20, seed=3, tune=True):
+def run_large(pkg, n=800, steps=20, seed=3, tune=True, fixed=None):
     """800^3 x 2 FP32 scaled-deviation form and FP64 path on the config-3
     workload (large_workload): autotuned shapes, graph-replayed step time,
     per-kernel times with the dominant kernel's roofline, and FP32 vs FP64
-    max relative error and mass drift per substrate after the run."""
+    max relative error and mass drift per substrate after the run.  fixed =
+    {"fp32": shape, "fp64": shape} (parse_tiles) pins a precision's shapes."""
     import importlib
     at = importlib.import_module("physicellx_amd.autotune")
     w = large_workload(n, seed)
@@ -474,7 +498,9 @@ def run_large(pkg, n=800, steps=20, seed=3, tune=True):
            "grid": [n, n, n], "substrates": ns, "agents": int(voxel.size), "steps": steps}
     envs = {}
     for prec in ("fp32", "fp64"):
-        tiles = at.autotune(pkg, n, n, n, ns, prec, 0.01) if tune else None
+        tiles = (fixed or {}).get(prec)
+        if tiles is None and tune:
+            tiles = at.autotune(pkg, n, n, n, ns, prec, 0.01)
         e = pkg.Microenvironment(n, n, n, ns, 20.0, prec)
         if tiles is not None:
             e.set_tiles(tiles["Lx"], tiles["Ly"], tiles["Lz"], tiles["rows_x"], tiles["cols_y"], tiles["cols_z"])
@@ -503,8 +529,9 @@ def run_large(pkg, n=800, steps=20, seed=3, tune=True):
                      "roofline": {"bound": "hbm", "kernel": f"sweep_{names[dom]}", "alg_bytes_per_launch": alg,
                                   "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                                   "frac": achieved / PEAK_HBM_GBS,
-                                  "traffic": load_traffic(f"sweep_{names[dom]}_{prec}_{n}")},
-                     "tiles": list(e.tiles()), "autotuned": tune}
+                                  "traffic": load_traffic(f"sweep_{names[dom]}_{prec}_{n}", e.tiles())},
+                     "tiles": list(e.tiles()), "autotuned": tune and (fixed or {}).get(prec) is None,
+                     "schedule": {k: (tiles or {}).get(k, 0) for k in ("slab", "substrate_streams", "x_grid")}}
         envs[prec] = e
     f32 = envs["fp32"].download("soa").reshape(ns, -1)
     f64 = envs["fp64"].download("soa").reshape(ns, -1)
@@ -537,6 +564,11 @@ def main():
     ap.add_argument("--no-seq", action="store_true", help="ensemble mode: skip the sequential plain solves")
     ap.add_argument("--tiles", default="", help="fixed FP32 shapes Lx,Ly,Lz,rows_x,cols_y,cols_z[,slab[,streams[,x_grid]]] "
                     "instead of autotuning (profiling runs)")
+    ap.add_argument("--tiles64", default="", help="fixed FP64 shapes, same fields as --tiles")
+    ap.add_argument("--repeats", type=int, default=20, help="ensemble mode: timed batched solves")
+    ap.add_argument("--batch-shape", default="", help="ensemble mode: fixed batch shapes "
+                    "Lx,Ly,Lz,cols_y,cols_z,col_threads,streams instead of autotuning (FP32)")
+    ap.add_argument("--batch-shape64", default="", help="ensemble mode: fixed FP64 batch shapes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -544,29 +576,30 @@ def main():
         sys.exit(2)
     pkg = load_pkg()
     if args.mode == "ensemble":
-        print(json.dumps({"ensemble": run_ensemble(pkg, sequential=not args.no_seq)}))
+        shapes = {p: tuple(int(x) for x in v.split(",")) for p, v in
+                  (("fp32", args.batch_shape), ("fp64", args.batch_shape64)) if v}
+        print(json.dumps({"ensemble": run_ensemble(pkg, sequential=not args.no_seq, repeats=args.repeats,
+                                                   shapes=shapes)}))
         return
     if args.mode == "serve":
         print(json.dumps({"serving": run_serving(pkg, n_req=args.requests)}))
         return
     if args.mode == "large":
-        print(json.dumps({"large": run_large(pkg)}))
+        fixed = {"fp32": parse_tiles(args.tiles), "fp64": parse_tiles(args.tiles64)}
+        print(json.dumps({"large": run_large(pkg, steps=args.steps, fixed=fixed)}))
         return
     n, ns = args.n, args.ns
     dt = 0.01
     w = make_workload(n, ns, args.agents, args.seed)
     updates = n ** 3 * ns
 
-    tiles = {"fp32": None, "fp64": None}
-    if args.tiles:
-        v = [int(x) for x in args.tiles.split(",")]
-        tiles["fp32"] = dict(Lx=v[0], Ly=v[1], Lz=v[2], rows_x=v[3], cols_y=v[4], cols_z=v[5],
-                             slab=v[6] if len(v) > 6 else 0, substrate_streams=v[7] if len(v) > 7 else 1,
-                             x_grid=v[8] if len(v) > 8 else 0)
-    elif not args.no_tune:
+    tiles = {"fp32": parse_tiles(args.tiles), "fp64": parse_tiles(args.tiles64)}
+    if not args.no_tune:
         import importlib
         at = importlib.import_module("physicellx_amd.autotune")
         for prec in ("fp32",) if args.no_fp64 else ("fp32", "fp64"):
+            if tiles[prec] is not None:
+                continue
             r = at.autotune(pkg, n, n, n, ns, prec, dt)
             tiles[prec] = r
             if args.verbose:
@@ -589,7 +622,8 @@ def main():
         drift = [float(abs(m32[d] - m64[d]) / abs(m64[d])) for d in range(ns)]
         res["fp64"] = dict(value=updates * args.steps / wall64, ms_per_step=1e3 * wall64 / args.steps,
                            eager_ms_per_step=1e3 * gwall64 / args.steps,
-                           kernel_ms=dict(x=ms64[0], y=ms64[1], z=ms64[2], agents=ms64[3]))
+                           kernel_ms=dict(x=ms64[0], y=ms64[1], z=ms64[2], agents=ms64[3]),
+                           tiles=list(env64.tiles()))
         res["fp32_vs_fp64"] = dict(max_rel_err_per_substrate=errs, mass_drift_per_substrate=drift,
                                    steps=args.warmup + 2 * args.steps,
                                    norm="max|f32 - f64| / max|f64| per substrate")
@@ -607,7 +641,7 @@ def main():
     sweeps = 2 if (slab32 and dom == 0) else 1
     alg_bytes = updates * 2 * 4 * sweeps  # one FP32 read + one write per voxel-substrate per sweep
     achieved = alg_bytes / (ms32[dom] * 1e-3) / 1e9
-    traffic = load_traffic(f"sweep_{names[dom]}_fp32_{n}")
+    traffic = load_traffic(f"sweep_{names[dom]}_fp32_{n}", tiles32)
     out = {
         "metric": METRIC,
         "value": updates * args.steps / wall32,
@@ -636,7 +670,7 @@ def main():
                       cols_y=tiles32[4], cols_z=tiles32[5], slab=slab32,
                       substrate_streams=(tiles["fp32"] or {}).get("substrate_streams", 1),
                       x_grid=(tiles["fp32"] or {}).get("x_grid", 0),
-                      autotuned=not args.no_tune),
+                      autotuned=not args.no_tune and not args.tiles),
         "aux_kernels": aux,
         "host_boundary": dict(host_boundary, note="field in host memory (FP64, reference layout): "
                               "upload + one step + download per step, PCIe included; not the value"),

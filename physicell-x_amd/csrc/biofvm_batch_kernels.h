@@ -121,12 +121,41 @@ __global__ __launch_bounds__(256) void kb_sweep_x(const EnvDesc<T>* __restrict__
 // (T >= 16 * PMAX).  The ticket of the next item is drawn while the current
 // one runs (two LDS slots, alternating), the coefficients are staged once per
 // item, and the loads of tile q+1 are issued before tile q is solved.
-// Dynamic LDS: (2*PMAX*L + T + PMAX) * sizeof(T) + 16, plus T*L*sizeof(T) for
-// the FP32 form's copy of the values (kb_col_lds).  Chunks of 128 bytes
+// Dynamic LDS: (2*PMAX*L + T + PMAX + 8) * sizeof(T) + 16, plus T*L*sizeof(T)
+// for the FP32 form's copy of the values (batch_col_lds).  Chunks of 128 bytes
 // or more take blocks of at most kb_col_max_threads(L * sizeof(T)) = 512
 // threads, which leaves them the registers to stay out of scratch.
 __host__ __device__ constexpr int kb_col_max_threads(int chunk_bytes) {
     return chunk_bytes >= 128 ? 512 : 1024;
+}
+
+// Column Dirichlet (D2 / D3) of the batch column sweeps: the node values
+// col_dirichlet writes, read from the substrate's seven slot values staged in
+// LDS (s_dv) with the face selection of dir_slot.  A line lying in a face
+// (lineb) is a Dirichlet line: every node takes the value of its own slot,
+// the two end nodes those of their slots; other lines only have their end
+// nodes on a face.  Nodes past the line end are overwritten too (they are
+// not stored), which leaves one position test per node instead of three.
+template <typename T, int L, int AXIS>
+__device__ __forceinline__ void col_dirichlet_staged(T (&x)[L], const T* s_dv, unsigned reg, int nx,
+                                                     int ny, int nz, int ci, int co, int s, int n,
+                                                     int p, int P) {
+    const bool lineb = AXIS == 1 ? (ci == 0 || ci == nx - 1 || co == 0 || co == nz - 1)
+                                 : (ci == 0 || ci == nx - 1 || co == 0 || co == ny - 1);
+    const int lo = AXIS == 1 ? dir_slot(ci, 0, co, nx, ny, nz, reg) : dir_slot(ci, co, 0, nx, ny, nz, reg);
+    const int hi = AXIS == 1 ? dir_slot(ci, n - 1, co, nx, ny, nz, reg) : dir_slot(ci, co, n - 1, nx, ny, nz, reg);
+    if (lineb) {
+        const T vin = s_dv[AXIS == 1 ? dir_slot(ci, -1, co, nx, ny, nz, reg) : dir_slot(ci, co, -1, nx, ny, nz, reg)];
+#pragma unroll
+        for (int m = 0; m < L; ++m) x[m] = vin;
+    }
+    if (p == 0) x[0] = s_dv[lo];
+    if (p == P - 1) {
+        const T v = s_dv[hi];
+#pragma unroll
+        for (int m = 0; m < L; ++m)
+            if (s + m == n - 1) x[m] = v;
+    }
 }
 
 template <typename T, int L, int AXIS, bool DEV>
@@ -141,7 +170,8 @@ __global__ __launch_bounds__(kb_col_max_threads(L * sizeof(T))) void kb_sweep_co
     T* s_z = s_cp + PMAX * L;
     T* s_g = s_z + blockDim.x;
     int* s_item = reinterpret_cast<int*>(s_g + PMAX);
-    T* s_v = reinterpret_cast<T*>(s_item + 4);  // FP32: blockDim.x * L values (stash of v)
+    T* s_dv = reinterpret_cast<T*>(s_item + 4);  // the item substrate's 7 Dirichlet slot values
+    T* s_v = s_dv + 8;                           // FP32: blockDim.x * L values (stash of v)
     const int t = threadIdx.x;
     const int n_items = *nitems;
     if (t == 0) s_item[0] = atomicAdd(ticket, 1);
@@ -159,13 +189,14 @@ __global__ __launch_bounds__(kb_col_max_threads(L * sizeof(T))) void kb_sweep_co
         const SweepArgs<T> a = args_of<T>(envs[w.env], AXIS);
         const int nx = a.nx, ny = a.ny, nz = a.nz;
         int n = AXIS == 1 ? ny : nz;
-        const int P = (n + L - 1) / L;
-        const int W = col_tile_width(blockDim.x, P);
-        const int lane = t % W;
-        const int p = t / W;
+        int P = (n + L - 1) / L;
+        int W = col_tile_width(blockDim.x, P);
+        int lane = t % W;
+        int p = t / W;
         const int d = w.d;
         stage_coefs<T>(s_inv, s_cp, a.inv + (long long)d * n, a.cp + (long long)d * n, n, P * L, t,
                        blockDim.x);
+        if (t < 7) s_dv[t] = a.dval[t * a.ns + d];
         const int ncols = AXIS == 1 ? nz * nx : ny * nx;
         const size_t stride = AXIS == 1 ? (size_t)nx : (size_t)nx * ny;
         T* ub = a.u + (long long)d * a.N;
@@ -199,11 +230,12 @@ __global__ __launch_bounds__(kb_col_max_threads(L * sizeof(T))) void kb_sweep_co
         fetch(act_n, base_n);
         if (t == 0) s_item[slot ^ 1] = nxt;
         for (int q = 0; q < w.ntiles; ++q) {
-            // the chunk position and line length are invariant over the
-            // item's tiles; laundering them per tile keeps the position tests
-            // (a scalar mask register pair each) from being hoisted out of
-            // the tile loop and spilling the scalar registers
-            asm volatile("" : "+v"(s), "+v"(n));
+            // the thread's place in the tile (chunk position, line length,
+            // chunk and column index, chunk count, tile width) is invariant
+            // over the item's tiles; laundering it per tile keeps the tests
+            // on it (a scalar mask register pair each) from being hoisted out
+            // of the tile loop and spilling the scalar registers
+            asm volatile("" : "+v"(s), "+v"(n), "+v"(p), "+v"(P), "+v"(lane), "+v"(W));
             T x[L];
 #pragma unroll
             for (int m = 0; m < L; ++m) x[m] = xn[m];
@@ -226,7 +258,7 @@ __global__ __launch_bounds__(kb_col_max_threads(L * sizeof(T))) void kb_sweep_co
                 __syncthreads();  // staging done / the previous tile's fold readers are done
                 chunk_solve<T, L>(x, act, p, P, lane, W, s_inv, s_cp, kd, s_z, s_g);
             }
-            if (act && dact) col_dirichlet<T, L, AXIS>(x, a, d, ci, co, s, n, p, P);
+            if (act && dact) col_dirichlet_staged<T, L, AXIS>(x, s_dv, a.reg, nx, ny, nz, ci, co, s, n, p, P);
             store_line_chunk<T, L>(x, ub + (size_t)base + (size_t)s * stride, stride, act ? n - s : 0);
         }
     }
