@@ -114,11 +114,17 @@ def time_gpu(env, dt, steps, warmup):
     return wall, ms, eager_wall
 
 
-def run_host_boundary(env, dt, updates, reps=3):
+def run_host_boundary(env, dt, updates, prec, reps=3, fix_basis=False):
     """The drop-in solver form, where the caller's field stays in host memory
     (FP64, reference layout, pageable): every step uploads it, runs one step
-    and downloads it again.  Median over reps of each part, PCIe included."""
+    and downloads it again.  Median over reps of each part, PCIe included.
+    FP32 downloads move 4-byte deviations; FP32 uploads do too once the basis
+    is fixed (fix_basis: the current basis is fixed first), otherwise they
+    move the FP64 field, from which the device derives the basis."""
     host = env.download("ref")
+    fp32 = prec == "fp32"
+    if fix_basis:
+        env.set_fp32_basis(*env.fp32_basis())
     parts = []
     for _ in range(reps + 1):  # the first round pays first-touch costs
         t0 = time.perf_counter()
@@ -134,7 +140,9 @@ def run_host_boundary(env, dt, updates, reps=3):
     total = float(np.median(p.sum(axis=1)))
     return {"upload_ms": float(np.median(p[:, 0])), "step_ms": float(np.median(p[:, 1])),
             "download_ms": float(np.median(p[:, 2])), "ms_per_step": total,
-            "value": updates / (total * 1e-3), "field_bytes_each_way": int(host.nbytes), "reps": reps}
+            "value": updates / (total * 1e-3), "field_bytes_each_way": int(host.nbytes),
+            "pcie_bytes_up": int(host.size * (4 if fp32 and fix_basis else 8)),
+            "pcie_bytes_down": int(host.size * (4 if fp32 else 8)), "reps": reps}
 
 
 def load_traffic(kernel_key, tiles):
@@ -627,12 +635,13 @@ def main():
         res["fp32_vs_fp64"] = dict(max_rel_err_per_substrate=errs, mass_drift_per_substrate=drift,
                                    steps=args.warmup + 2 * args.steps,
                                    norm="max|f32 - f64| / max|f64| per substrate")
-        host_boundary = {"fp64": run_host_boundary(env64, dt, updates)}
+        host_boundary = {"fp64": run_host_boundary(env64, dt, updates, "fp64")}
         env64.close()
     else:
         host_boundary = {}
     aux = run_aux(env32, w, updates)
-    host_boundary["fp32"] = run_host_boundary(env32, dt, updates)
+    host_boundary["fp32"] = run_host_boundary(env32, dt, updates, "fp32")
+    host_boundary["fp32_fixed_basis"] = run_host_boundary(env32, dt, updates, "fp32", fix_basis=True)
     env32.close()
 
     slab32 = tiles["fp32"]["slab"] if tiles["fp32"] else 0

@@ -1065,6 +1065,42 @@ int bfa_upload(bfa_env* e, const double* field, int layout) {
         HIP_TRY(e, copy_h2d(e->u, field, total * 8, e->stream));
         return BFA_OK;
     }
+    if (e->prec == BFA_FP32 && e->basis_fixed) {
+        // the basis is known before the field is read: the host copy threads
+        // form the FP32 deviations, and half the bytes cross PCIe
+        e->pre_valid = false;
+        std::vector<double> p(5 * (size_t)e->ns, 0.0), rs(e->ns);
+        for (int d = 0; d < e->ns; ++d) {
+            rs[d] = 1.0 / e->s[d];
+            p[d] = e->coef_set ? e->D[d] : 0.0;
+            p[e->ns + d] = e->coef_set ? e->decay[d] : 0.0;
+            p[2 * e->ns + d] = e->b[d];
+            p[3 * e->ns + d] = e->s[d];
+            p[4 * e->ns + d] = rs[d];
+        }
+        hipError_t st = hipMemcpyAsync(e->params, p.data(), p.size() * 8, hipMemcpyHostToDevice, e->stream);
+        if (st != hipSuccess) return fail(e, BFA_ERR_HIP, "params copy failed: %s", hipGetErrorString(st));
+        const bool ref = layout == BFA_LAYOUT_REF;
+        float* st32 = nullptr;
+        if (ref) {
+            int rc = dalloc(e, &st32, total * 4);
+            if (rc) return rc;
+        }
+        st = copy_h2d_to_f32(ref ? st32 : (float*)e->u, field, total, e->b.data(), rs.data(), e->ns, e->N, ref,
+                             e->stream);
+        if (st == hipSuccess && ref) {
+            dim3 grid((unsigned)(((long long)e->nz * e->ns + 31) / 32), (unsigned)((e->nx + 31) / 32),
+                      (unsigned)e->ny);
+            hipLaunchKernelGGL(bfa::k_ref32_to_soa, grid, dim3(32, 8), 0, e->stream, st32, (float*)e->u, e->nx,
+                               e->ny, e->nz, e->ns);
+            st = hipGetLastError();
+        }
+        const hipError_t ss = hipStreamSynchronize(e->stream);
+        if (st == hipSuccess) st = ss;
+        dfree(st32);
+        if (st != hipSuccess) return fail(e, BFA_ERR_HIP, "upload failed: %s", hipGetErrorString(st));
+        return BFA_OK;
+    }
     int rc = dalloc(e, &stage, total * 8);
     if (rc) return rc;
     auto cleanup = [&](int code) {
@@ -1202,6 +1238,29 @@ int bfa_download(bfa_env* e, double* field, int layout) {
         HIP_TRY(e, copy_d2h(field, e->u, total * 8, e->stream));
         return BFA_OK;
     }
+    if (e->prec == BFA_FP32) {
+        // the FP32 deviations cross PCIe (half the bytes) and the host copy
+        // threads expand them to u = b + s*v
+        const bool ref = layout == BFA_LAYOUT_REF;
+        float* st32 = nullptr;
+        hipError_t st = hipSuccess;
+        if (ref) {
+            int rc = dalloc(e, &st32, total * 4);
+            if (rc) return rc;
+            dim3 grid((unsigned)(((long long)e->nz * e->ns + 31) / 32), (unsigned)((e->nx + 31) / 32),
+                      (unsigned)e->ny);
+            hipLaunchKernelGGL(bfa::k_soa_to_ref32, grid, dim3(32, 8), 0, e->stream, (const float*)e->u, st32,
+                               e->nx, e->ny, e->nz, e->ns);
+            st = hipGetLastError();
+        }
+        if (st == hipSuccess)
+            st = copy_d2h_from_f32(field, ref ? st32 : (const float*)e->u, total, e->b.data(), e->s.data(), e->ns,
+                                   e->N, ref, e->stream);
+        (void)hipStreamSynchronize(e->stream);
+        dfree(st32);
+        if (st != hipSuccess) return fail(e, BFA_ERR_HIP, "download failed: %s", hipGetErrorString(st));
+        return BFA_OK;
+    }
     double* stage = nullptr;
     int rc = dalloc(e, &stage, total * 8);
     if (rc) return rc;
@@ -1219,23 +1278,11 @@ int bfa_download(bfa_env* e, double* field, int layout) {
     double* dbs = nullptr;
     if ((rc = dalloc(e, &dbs, bs.size() * 8))) return cleanup(rc);
     hipError_t st = hipMemcpyAsync(dbs, bs.data(), bs.size() * 8, hipMemcpyHostToDevice, e->stream);
-    if (st == hipSuccess) {
-        if (layout == BFA_LAYOUT_REF) {
-            dim3 grid((unsigned)(((long long)e->nz * e->ns + 31) / 32), (unsigned)((e->nx + 31) / 32),
-                      (unsigned)e->ny);
-            if (e->prec == BFA_FP32)
-                hipLaunchKernelGGL(bfa::k_soa_to_ref<float>, grid, dim3(32, 8), 0, e->stream,
-                                   (const float*)e->u, stage, e->nx, e->ny, e->nz, e->ns, dbs,
-                                   dbs + e->ns);
-            else
-                hipLaunchKernelGGL(bfa::k_soa_to_ref<double>, grid, dim3(32, 8), 0, e->stream,
-                                   (const double*)e->u, stage, e->nx, e->ny, e->nz, e->ns, dbs,
-                                   dbs + e->ns);
-        } else {
-            dim3 grid((unsigned)std::min<long long>(4096, (e->N + 255) / 256), (unsigned)e->ns);
-            hipLaunchKernelGGL(bfa::k_soa_to_f64<float>, grid, dim3(256), 0, e->stream,
-                               (const float*)e->u, stage, e->N, dbs, dbs + e->ns);
-        }
+    if (st == hipSuccess) {  // FP64, reference layout
+        dim3 grid((unsigned)(((long long)e->nz * e->ns + 31) / 32), (unsigned)((e->nx + 31) / 32),
+                  (unsigned)e->ny);
+        hipLaunchKernelGGL(bfa::k_soa_to_ref<double>, grid, dim3(32, 8), 0, e->stream, (const double*)e->u,
+                           stage, e->nx, e->ny, e->nz, e->ns, dbs, dbs + e->ns);
         st = hipGetLastError();
     }
     if (st == hipSuccess) st = copy_d2h(field, stage, total * 8, e->stream);

@@ -309,6 +309,56 @@ __global__ __launch_bounds__(256) void k_soa_to_ref(const T* __restrict__ in,
 }
 
 // Device-layout FP64 to T and back, elementwise (substrate d = blockIdx.y).
+// The same transposes on FP32 deviations that the host converts itself
+// (transfers of FP32 fields at half the PCIe bytes): no arithmetic here.
+__global__ __launch_bounds__(256) void k_ref32_to_soa(const float* __restrict__ in, float* __restrict__ out,
+                                                      int nx, int ny, int nz, int ns) {
+    __shared__ float tile[32][33];
+    const long long Q = (long long)nz * ns;
+    const long long q0 = (long long)blockIdx.x * 32;
+    const int i0 = blockIdx.y * 32;
+    const int j = blockIdx.z;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + r;
+        const long long q = q0 + tx;
+        if (i < nx && q < Q) tile[r][tx] = in[((long long)i * ny + j) * Q + q];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const long long q = q0 + r;
+        const int i = i0 + tx;
+        if (i < nx && q < Q) {
+            const int k = (int)(q / ns), d = (int)(q - (long long)k * ns);
+            out[(((long long)d * nz + k) * ny + j) * nx + i] = tile[tx][r];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_soa_to_ref32(const float* __restrict__ in, float* __restrict__ out,
+                                                      int nx, int ny, int nz, int ns) {
+    __shared__ float tile[32][33];
+    const long long Q = (long long)nz * ns;
+    const long long q0 = (long long)blockIdx.x * 32;
+    const int i0 = blockIdx.y * 32;
+    const int j = blockIdx.z;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    for (int r = ty; r < 32; r += 8) {
+        const long long q = q0 + r;
+        const int i = i0 + tx;
+        if (i < nx && q < Q) {
+            const int k = (int)(q / ns), d = (int)(q - (long long)k * ns);
+            tile[r][tx] = in[(((long long)d * nz + k) * ny + j) * nx + i];
+        }
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + r;
+        const long long q = q0 + tx;
+        if (i < nx && q < Q) out[((long long)i * ny + j) * Q + q] = tile[tx][r];
+    }
+}
+
 template <typename T>
 __global__ void k_soa_from_f64(const double* __restrict__ in, T* __restrict__ out, long long N,
                                const double* __restrict__ b, const double* __restrict__ rs) {

@@ -2,7 +2,9 @@
 // (physicell-x_amd/csrc/biofvm_transfer.inc): for crews of 1 to 16 threads
 // and sizes around the 4 KiB slice alignment, one crew reused for every
 // copy reproduces the source byte for byte and writes nothing outside the
-// destination range.  No GPU call is made.  Exit status 0 on success.
+// destination range; element jobs visit every element once; the substrate
+// runs of the converting transfers give e % ns / e / N.  No GPU call is
+// made.  Exit status 0 on success.
 #include "biofvm_transfer.inc"
 
 #include <cstdio>
@@ -32,6 +34,54 @@ int main() {
             }
         }
     }
-    std::printf("copy crew: %d copies, %d failures\n", copies, failures);
+    // element jobs: every element of [0, count) is visited exactly once
+    int jobs = 0;
+    for (int threads : {1, 2, 3, 8, 16}) {
+        CopyCrew crew(threads);
+        for (size_t count : {(size_t)0, (size_t)1, (size_t)1023, (size_t)1024, (size_t)1025, (size_t)5000,
+                             ((size_t)3 << 20) + 5}) {
+            std::vector<int> hits(count, 0);
+            crew.each(count, [&](size_t a, size_t b) {
+                for (size_t q = a; q < b; ++q) ++hits[q];
+            });
+            ++jobs;
+            for (size_t q = 0; q < count; ++q)
+                if (hits[q] != 1) {
+                    std::printf("FAIL: job of %zu elements, %d threads: element %zu visited %d times\n", count,
+                                threads, q, hits[q]);
+                    ++failures;
+                    break;
+                }
+        }
+    }
+    // runs of the converting transfers, from any start: contiguous, and the
+    // substrate of each element is e % ns (reference layout) or e / N
+    // (device layout)
+    int walks = 0;
+    for (int ns : {1, 2, 3, 7})
+        for (long long N : {1LL, 5LL, 13LL})
+            for (bool ref : {true, false}) {
+                const size_t total = (size_t)N * ns;
+                for (size_t e0 = 0; e0 < total; ++e0) {
+                    size_t next = 0;
+                    bool ok = true;
+                    each_run(e0, total - e0, ns, N, ref, [&](size_t q, size_t m, int d) {
+                        ok = ok && q == next && m > 0;
+                        for (size_t k = 0; k < m; ++k) {
+                            const size_t e = e0 + q + k;
+                            const int want = ref ? (int)(e % ns) : (int)(e / N);
+                            ok = ok && (ref ? d + (int)k : d) == want;
+                        }
+                        next = q + m;
+                    });
+                    if (!ok || next != total - e0) {
+                        std::printf("FAIL: runs ns=%d N=%lld ref=%d from %zu\n", ns, N, (int)ref, e0);
+                        ++failures;
+                    }
+                    ++walks;
+                }
+            }
+    std::printf("copy crew: %d copies, %d element jobs, %d substrate runs, %d failures\n", copies, jobs, walks,
+                failures);
     return failures != 0;
 }

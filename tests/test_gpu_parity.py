@@ -229,6 +229,52 @@ def test_layout_round_trips(pkg, precision):
         env.close()
 
 
+def deviation_round_trip(u, b, s, ns, ref):
+    """u -> v = float32((u - b) * (1/s)) -> b + s*v: what an FP32 upload and
+    download must give bit for bit (s a power of two, so s*v is exact and
+    the fused multiply-add rounds once, like the plain sum)."""
+    d = np.arange(u.size) % ns if ref else np.arange(u.size) // (u.size // ns)
+    v = ((u - b[d]) * (1.0 / s[d])).astype(np.float32)
+    return v.astype(np.float64) * s[d] + b[d]
+
+
+@pytest.mark.parametrize("shape", [(19, 22, 25, 3), (211, 197, 163, 4)])
+def test_fp32_transfers_round_deviations_exactly(pkg, shape):
+    """FP32 downloads move the deviations over PCIe and expand them on the
+    host; with a fixed basis uploads form them on the host too.  Both give
+    the device conversion's values bit for bit, in both layouts, for a field
+    below the staging threshold and one of four 64 MiB chunks."""
+    nx, ny, nz, ns = shape
+    rng = np.random.default_rng(78)
+    u = rng.uniform(1.0, 50.0, nx * ny * nz * ns)
+    soa = u.reshape(nx, ny, nz, ns).transpose(3, 2, 1, 0).ravel()
+    env = pkg.Microenvironment(nx, ny, nz, ns, 20.0, "fp32")
+    try:
+        env.set_coefficients(np.full(ns, 1e3), np.full(ns, 0.1))
+        env.upload(u, "ref")  # basis from the field, deviations formed on the device
+        b, s = env.fp32_basis()
+        dev = env.download("soa")
+        assert np.array_equal(env.download("ref"), deviation_round_trip(u, b, s, ns, True))
+        assert np.array_equal(dev, deviation_round_trip(soa, b, s, ns, False))
+        # the same basis fixed: the host forms the deviations, same device field
+        env.set_fp32_basis(b, s)
+        env.upload(u, "ref")
+        assert np.array_equal(env.download("soa"), dev)
+        env.upload(soa, "soa")
+        assert np.array_equal(env.download("soa"), dev)
+        # another fixed basis
+        b2 = b + 0.75
+        s2 = s * 4.0
+        env.set_fp32_basis(b2, s2)
+        env.upload(u, "ref")
+        assert np.array_equal(env.download("ref"), deviation_round_trip(u, b2, s2, ns, True))
+        env.upload(soa, "soa")
+        assert np.array_equal(env.download("soa"), deviation_round_trip(soa, b2, s2, ns, False))
+        assert np.array_equal(env.fp32_basis()[0], b2)
+    finally:
+        env.close()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_staged_transfers_of_large_fields(pkg, precision):
     """Fields of 16 MiB and more move through the pinned chunk pipeline

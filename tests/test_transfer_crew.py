@@ -1,6 +1,8 @@
 """The host-thread crew of the pinned transfer pipeline, on the CPU: copies
 of many sizes through crews of 1-16 threads are byte-exact and stay inside
-the destination (driven by tests/cpp/copy_crew_test, no GPU work)."""
+the destination, element jobs visit every element once, and the substrate
+runs of the FP32 converting transfers follow the field layouts (driven by
+tests/cpp/copy_crew_test, no GPU work)."""
 import os
 import subprocess
 
