@@ -61,7 +61,7 @@ def _step_ms(env, dt, reps):
     return (time.perf_counter() - t0) / n * 1e3
 
 
-def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=3, cache_path=None,
+def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=6, cache_path=None,
              verbose=False):
     """Return {"Lx", "rows_x", "Ly", "cols_y", "Lz", "cols_z", "x_grid", "slab",
     "substrate_streams", "ms", "trials"}."""
@@ -86,20 +86,36 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=3, cache_path=
             env.upload(20.0 + rng.standard_normal(nx * ny * nz * ns), "soa")
         else:  # kernel times do not depend on the values; skip a host field of this size
             env.fill(np.full(ns, 21.0))
+        # wave-local x sweep: chunk length and resident blocks per CU together
+        # (the best block count differs between chunk lengths)
         best_x = None
         for L in X_WAVE_L:
+            env.set_x_grid(0)
             ms = _time(env, dt, reps, (L, 0, 0, 0, 0, 0))
-            if ms is not None:
-                trials.append({"axis": "x", "form": "wave", "L": L, "ms": ms[0]})
-                if best_x is None or ms[0] < best_x[0]:
-                    best_x = (ms[0], L, 0)
+            if ms is None:
+                continue
+            trials.append({"axis": "x", "form": "wave", "L": L, "blocks_per_cu": 0, "ms": ms[0]})
+            if best_x is None or ms[0] < best_x[0]:
+                best_x = (ms[0], L, 0, 0)
+            resident = env.x_grid()[1]
+            for g in X_GRID:
+                if g >= resident:
+                    break
+                env.set_x_grid(g)
+                env.time_kernels(dt, 1)
+                ms = env.time_kernels(dt, reps)
+                trials.append({"axis": "x", "form": "wave", "L": L, "blocks_per_cu": g, "ms": ms[0]})
+                if ms[0] < best_x[0]:
+                    best_x = (ms[0], L, 0, g)
+        env.set_x_grid(0)
         for R, L in X_LDS:
             ms = _time(env, dt, reps, (L, 0, 0, R, 0, 0))
             if ms is not None:
                 trials.append({"axis": "x", "form": "lds", "R": R, "L": L, "ms": ms[0]})
                 if best_x is None or ms[0] < best_x[0]:
-                    best_x = (ms[0], L, R)
+                    best_x = (ms[0], L, R, 0)
         best_y = best_z = None
+        env.set_x_grid(best_x[3])
         for TX in COL_TX:
             for L in COL_L:
                 ms = _time(env, dt, reps, (best_x[1], L, L, best_x[2], TX, TX))
@@ -112,20 +128,7 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=3, cache_path=
                     best_z = (ms[2], L, TX)
         shape = (best_x[1], best_y[1], best_z[1], best_x[2], best_y[2], best_z[2])
         env.set_tiles(*shape)
-        # blocks per CU of the wave-local x sweep (0: as many as fit)
-        best_grid = (best_x[0], 0)
-        if best_x[2] == 0:
-            resident = env.x_grid()[1]
-            for g in X_GRID:
-                if g >= resident:
-                    break
-                env.set_x_grid(g)
-                env.time_kernels(dt, 1)
-                ms = env.time_kernels(dt, reps)
-                trials.append({"axis": "x", "form": "wave", "L": best_x[1], "blocks_per_cu": g, "ms": ms[0]})
-                if ms[0] < best_grid[0]:
-                    best_grid = (ms[0], g)
-            env.set_x_grid(best_grid[1])
+        best_grid = (best_x[0], best_x[3])
         # schedules, by graph-replayed step time: whole-volume sweeps, the two
         # substrate halves on parallel streams, x/y k-slabs
         cands = [(0, 1)]

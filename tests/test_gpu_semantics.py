@@ -253,3 +253,53 @@ def test_dt_caching_keeps_the_first_coefficients(pkg, precision, tol):
         assert relmax(env.download(), v) <= tol
     finally:
         env.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_setters_right_after_asynchronous_steps(pkg, precision):
+    """bfa_step only enqueues its graph launches.  Calls that drop the captured
+    graph (new coefficients, a new dt, new kernel shapes, new agents) may come
+    while those launches still run; the graph is drained before it is
+    destroyed, so the result equals a run synchronised before every call.
+    The batch's table regrowth (a member added after a step) likewise."""
+    p = make_problem(40, 36, 44, 2, seed=91, n_agents=300, dirichlet="mixed", collide=3)
+
+    def run(sync):
+        env = gpu_env(pkg, p, precision)
+        try:
+            calls = [lambda: env.set_coefficients(p.D * 1.5, p.decay),
+                     lambda: env.step(p.dt * 2, 1),
+                     lambda: env.set_tiles(8, 8, 8, 0, 16, 16),
+                     lambda: env.set_agents(p.voxel[::-1].copy(), p.c1[::-1], p.c2[::-1], p.e2[::-1])]
+            for call in calls:
+                env.step(p.dt, 15)
+                if sync:
+                    env.synchronize()
+                call()
+            env.step(p.dt, 5)
+            return env.download()
+        finally:
+            env.close()
+
+    assert np.array_equal(run(False), run(True))
+
+    def run_batch(sync):
+        b = pkg.Batch(precision)
+        try:
+            q = make_problem(30, 28, 26, 2, seed=92, n_agents=100)
+            ids = []
+            for _ in range(3):
+                mid = b.add(q.nx, q.ny, q.nz, q.ns, q.dx, q.D, q.decay, q.active, q.reg, q.fv)
+                b.upload(mid, q.u0)
+                b.set_agents(mid, q.voxel, q.c1, q.c2, q.e2)
+                ids.append(mid)
+                b.step(q.dt, 15)
+                if sync:
+                    b.synchronize()
+            b.step(q.dt, 5)
+            return [b.download(m) for m in ids]
+        finally:
+            b.close()
+
+    for a, c in zip(run_batch(False), run_batch(True)):
+        assert np.array_equal(a, c)
