@@ -254,7 +254,8 @@ int bfa_time_kernels(bfa_env *env, double dt, int reps, double *ms);
 int bfa_copy_kernel(size_t bytes, int reps, double *ms);
 
 /* Kernel shape selection, used by the autotuner: the chunk length L of the
- * in-register line solve per axis (4, 8, 16, 32 or 64; 0 = automatic); the
+ * in-register line solve per axis (4, 8, 16, 32 or 64, and 25 for y and z;
+ * 0 = automatic); the
  * x-sweep form: rows_x == 0 selects the wave-local x sweep (a row's chunks
  * live in one wavefront), rows_x in {1,2,4,...,64} the LDS-tiled x sweep
  * with that many rows per tile; cols_y / cols_z, the x-columns per y / z

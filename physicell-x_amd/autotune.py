@@ -9,8 +9,10 @@ sweep kernels on the GPU (HIP events around each launch, via
   form with R rows per tile and chunk length L; for the wave-local form, the
   number of resident blocks per compute unit (fewer waves in flight can
   stream faster);
-* y and z sweeps: TX in {16, 32, 64} x-columns per block and chunk length L,
-  with TX * ceil(n/L) <= 1024 threads, chosen per axis;
+* y and z sweeps: TX in {16, 32, 64} x-columns per block and chunk length L
+  in {4, 8, 16, 25, 32}, with TX * ceil(n/L) <= 1024 threads, chosen per axis
+  (25-node chunks cut a 400-node line into 16 chunks: 32 columns then fill
+  eight whole wavefronts with no padded nodes);
 * schedule: whole-volume sweeps, the two substrate halves swept on parallel
   streams, or x/y sweeps in k-slabs of a few planes (chosen by graph-replayed
   whole-step time).
@@ -28,7 +30,7 @@ import numpy as np
 X_WAVE_L = (4, 8, 16, 32)
 X_LDS = [(R, L) for R in (8, 16, 32) for L in (8, 16, 32)]
 COL_TX = (16, 32, 64)
-COL_L = (4, 8, 16, 32)
+COL_L = (4, 8, 16, 25, 32)
 SLAB_BYTES = (8 << 20, 16 << 20, 32 << 20, 64 << 20, 128 << 20)
 X_GRID = (2, 3, 4, 5, 6, 8)
 

@@ -30,6 +30,8 @@ thread_local std::string g_create_error;
 constexpr int kMaxEdge = 1024;
 constexpr int kMaxSubstrates = 64;
 constexpr int kLs[] = {4, 8, 16, 32, 64};
+// column sweeps also take 25-node chunks (16 chunks of a 400-node line)
+constexpr int kColLs[] = {4, 8, 16, 25, 32, 64};
 
 }  // namespace
 
@@ -233,7 +235,7 @@ int choose_tiles(bfa_env* e) {
         }
         if (TX != 16 && TX != 32 && TX != 64)
             return fail(e, BFA_ERR_ARG, "columns per block must be 16, 32 or 64, got %d", TX);
-        if (L == 0 || std::find(std::begin(kLs), std::end(kLs), L) == std::end(kLs))
+        if (L == 0 || std::find(std::begin(kColLs), std::end(kColLs), L) == std::end(kColLs))
             return fail(e, BFA_ERR_ARG, "axis %d of length %d: unsupported chunk length %d", a, n, L);
         if (((n + L - 1) / L) * TX > 1024)
             return fail(e, BFA_ERR_ARG, "axis %d of length %d: %d chunks of %d x %d columns exceed 1024 threads",
@@ -432,6 +434,10 @@ int raise_lds_limits(bfa_env* e) {
     if (e->L[1] == Lv) fy = BFA_COLFN(Lv, 1);                                 \
     if (e->L[2] == Lv) fz = BFA_COLFN(Lv, 2);
     BFA_PICK(4) BFA_PICK(8) BFA_PICK(16) BFA_PICK(32) BFA_PICK(64)
+    // column-only chunk length: 25 nodes make 16 chunks of a 400-node line,
+    // so 32 columns fill eight whole wavefronts with no padded nodes
+    if (e->L[1] == 25) fy = BFA_COLFN(25, 1);
+    if (e->L[2] == 25) fz = BFA_COLFN(25, 2);
 #undef BFA_PICK
 #undef BFA_COLFN
     const size_t need[3] = {e->R ? lds_x(e, e->R, e->L[0]) : 0, lds_col(e, e->ny, e->L[1], e->TX[1]),
@@ -480,6 +486,7 @@ int launch_sweep(bfa_env* e, int axis, hipStream_t st, const Win& w) {
             case 4: return launch_col_L<T, 4, 1>(e, st, w);
             case 8: return launch_col_L<T, 8, 1>(e, st, w);
             case 16: return launch_col_L<T, 16, 1>(e, st, w);
+            case 25: return launch_col_L<T, 25, 1>(e, st, w);
             case 32: return launch_col_L<T, 32, 1>(e, st, w);
             case 64: return launch_col_L<T, 64, 1>(e, st, w);
         }
@@ -488,6 +495,7 @@ int launch_sweep(bfa_env* e, int axis, hipStream_t st, const Win& w) {
             case 4: return launch_col_L<T, 4, 2>(e, st, w);
             case 8: return launch_col_L<T, 8, 2>(e, st, w);
             case 16: return launch_col_L<T, 16, 2>(e, st, w);
+            case 25: return launch_col_L<T, 25, 2>(e, st, w);
             case 32: return launch_col_L<T, 32, 2>(e, st, w);
             case 64: return launch_col_L<T, 64, 2>(e, st, w);
         }

@@ -374,7 +374,8 @@ def test_changing_dt_recomputes_coefficients(pkg):
 
 TILES = [(4, 4, 4, 1, 64, 64), (8, 16, 32, 8, 32, 16), (32, 8, 16, 32, 16, 64),
          (16, 32, 4, 64, 64, 32), (4, 8, 8, 0, 16, 16), (8, 4, 16, 0, 32, 64),
-         (16, 16, 4, 0, 64, 16), (32, 32, 32, 0, 32, 32)]
+         (16, 16, 4, 0, 64, 16), (32, 32, 32, 0, 32, 32), (8, 25, 25, 0, 32, 16),
+         (16, 25, 8, 0, 64, 32)]
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
@@ -392,6 +393,27 @@ def test_tile_shapes_give_the_same_answer(pkg, tiles, precision):
     finally:
         base.close()
         other.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("fp32", FP32_TOL)])
+def test_column_chunks_of_25_nodes_against_oracle(pkg, precision, tol):
+    """25-node column chunks: a 400-node y line in 16 whole chunks, a 57-node z
+    line in two whole chunks and a partial one, with Dirichlet and agents."""
+    p = make_problem(12, 400, 57, 2, seed=57, n_agents=300, dirichlet="mixed", collide=4)
+    orc = oracle_for(p)
+    ag = agents_for(p)
+    env = gpu_env(pkg, p, precision)
+    try:
+        env.set_tiles(8, 25, 25, 0, 32, 16)
+        assert env.tiles()[1:3] == (25, 25)
+        u = p.u0.copy()
+        for s in range(3):
+            orc.diffusion_step(u, p.dt)
+            orc.secretion_uptake(u, ag)
+            env.step(p.dt, 1)
+            assert relmax(env.download(), u) <= tol, s
+    finally:
+        env.close()
 
 
 @pytest.mark.parametrize("nx", [1, 3, 63, 64, 65, 130, 257, 1000])
