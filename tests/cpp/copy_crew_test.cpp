@@ -81,7 +81,53 @@ int main() {
                     ++walks;
                 }
             }
-    std::printf("copy crew: %d copies, %d element jobs, %d substrate runs, %d failures\n", copies, jobs, walks,
-                failures);
+    // the conversions of the FP32 transfers against their element-wise
+    // definitions, bit for bit: u = fma(v, s[d], b[d]) and
+    // v = (float)((u - b[d]) * rs[d]), for slices at any offset, length and
+    // destination alignment, in both layouts
+    int convs = 0;
+    std::uniform_real_distribution<double> ud(-50.0, 50.0);
+    for (int ns : {1, 2, 3, 4, 7})
+        for (long long N : {5LL, 13LL, 1000LL})
+            for (bool ref : {true, false}) {
+                const size_t total = (size_t)N * ns;
+                std::vector<double> s(ns), b(ns), rs(ns);
+                for (int d = 0; d < ns; ++d) {
+                    s[d] = std::ldexp(1.0, (int)(rng() % 9) - 4) * (d % 2 ? 1.0 : 1.5);
+                    b[d] = ud(rng);
+                    rs[d] = 1.0 / s[d];
+                }
+                std::vector<float> v(total);
+                std::vector<double> u(total);
+                for (size_t q = 0; q < total; ++q) {
+                    v[q] = (float)ud(rng);
+                    u[q] = ud(rng);
+                }
+                for (size_t e0 : {(size_t)0, (size_t)1, (size_t)3, total / 3})
+                    for (size_t pad : {(size_t)0, (size_t)1, (size_t)2, (size_t)3}) {
+                        if (e0 > total) continue;
+                        const size_t n = total - e0;
+                        std::vector<double> out(n + pad + 8, -1.0);
+                        expand_slice(out.data() + pad, v.data() + e0, e0, n, s.data(), b.data(), ns, N, ref);
+                        std::vector<float> back(n + 8, -1.0f);
+                        compress_slice(back.data(), u.data() + e0, e0, n, b.data(), rs.data(), ns, N, ref);
+                        bool ok = out[n + pad] == -1.0;
+                        for (size_t q = 0; q < n && ok; ++q) {
+                            const size_t e = e0 + q;
+                            const int d = ref ? (int)(e % ns) : (int)(e / N);
+                            const double want = std::fma((double)v[e], s[d], b[d]);
+                            const float wantf = (float)((u[e] - b[d]) * rs[d]);
+                            ok = std::memcmp(&out[pad + q], &want, 8) == 0 && std::memcmp(&back[q], &wantf, 4) == 0;
+                        }
+                        if (!ok) {
+                            std::printf("FAIL: conversion ns=%d N=%lld ref=%d from %zu, pad %zu\n", ns, N, (int)ref, e0,
+                                        pad);
+                            ++failures;
+                        }
+                        ++convs;
+                    }
+            }
+    std::printf("copy crew: %d copies, %d element jobs, %d substrate runs, %d conversions, %d failures\n", copies,
+                jobs, walks, convs, failures);
     return failures != 0;
 }
