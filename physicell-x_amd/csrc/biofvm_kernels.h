@@ -423,20 +423,21 @@ __device__ __forceinline__ void load_chunk(T (&x)[L], const T* __restrict__ src)
 
 // Overwrite the Dirichlet nodes of the chunk [s, s+L) of a row of n nodes:
 // the ends take v_lo / v_hi, and when the whole row is on a face (rowb) the
-// interior takes v_in.  Static register indices only.
+// interior takes v_in.  Static register indices only.  Nodes past the row end
+// take v_in too on a face row: no caller reads or stores them (the solves run
+// them on zero coefficients and the stores stop at the row end), and one
+// position test per node remains instead of three.
 template <typename T, int L>
 __device__ __forceinline__ void dirichlet_row(T (&x)[L], int s, int n, bool rowb, T v_lo,
                                               T v_hi, T v_in) {
+    if (rowb) {
 #pragma unroll
-    for (int m = 0; m < L; ++m) {
-        const int pos = s + m;
-        if (pos < n) {
-            T v = rowb ? v_in : x[m];
-            v = pos == 0 ? v_lo : v;
-            v = pos == n - 1 ? v_hi : v;
-            x[m] = v;
-        }
+        for (int m = 0; m < L; ++m) x[m] = v_in;
     }
+    if (s == 0) x[0] = v_lo;
+#pragma unroll
+    for (int m = 0; m < L; ++m)
+        if (s + m == n - 1) x[m] = v_hi;
 }
 
 template <typename T>
