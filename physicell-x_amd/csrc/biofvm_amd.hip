@@ -183,7 +183,8 @@ int pick_L(int n, int pmax) {
 
 size_t lds_col(const bfa_env* e, int n, int L, int TX) {
     const int P = (n + L - 1) / L;
-    const size_t stash = e->prec == BFA_FP32 ? (size_t)TX * P * L : 0;  // the FP32 form's copy of the values
+    // the FP32 form's copy of the values, when the kernel keeps it in LDS
+    const size_t stash = e->prec == BFA_FP32 && !bfa::col_stash_in_regs(L) ? (size_t)TX * P * L : 0;
     return ((size_t)(2 * P * L + TX * P + P) + stash) * e->tsize;
 }
 

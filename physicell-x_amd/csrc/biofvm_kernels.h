@@ -865,9 +865,16 @@ __device__ __forceinline__ void store_line_chunk(const T (&x)[L], T* line, size_
     }
 }
 
+// Where the FP32 column sweep keeps a chunk's values during the increment
+// solve: in registers for chunks of 32 or more nodes (their register copy
+// still fits, and the LDS copy of a whole block would leave room for fewer
+// blocks: at 800^3, y 1.60 against 1.68 ms, z 1.86 against 1.94 ms), in LDS
+// below (at 400^3 with 25-node chunks, y 0.371 against 0.389 ms).
+__host__ __device__ constexpr bool col_stash_in_regs(int L) { return L >= 32; }
+
 // blockDim.x = TX*P with P = ceil(n/L); grid = (ceil(ncols/TX), ns).
 // Dynamic LDS: (2*P*L + TX*P + P) * sizeof(T), plus TX*P*L * sizeof(T) for the
-// FP32 form's copy of the values.
+// FP32 form's copy of the values when it is kept in LDS.
 template <typename T, int L, int AXIS, bool DEV, int TX>
 __global__ __launch_bounds__(1024) void k_sweep_col(SweepArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -908,15 +915,27 @@ __global__ __launch_bounds__(1024) void k_sweep_col(SweepArgs<T> a) {
     if (DEV) {
         // delta form: the chunk's outer neighbours come from the field (the
         // neighbouring chunks' threads load them too, so they hit in cache);
-        // the values wait in LDS for x = v + delta
+        // the values wait for x = v + delta in LDS, or with chunks of 32 in
+        // registers (col_stash_in_regs)
         const T vl = (act && s > 0) ? *(line - stride) : T(0);
         const T vr = (act && s + L < n) ? line[(size_t)L * stride] : T(0);
         T* s_v = s_g + P;
-        col_stash<T, L>(s_v, x, p, lane, TX);
+        T vsave[col_stash_in_regs(L) ? L : 1];
+        if constexpr (col_stash_in_regs(L)) {
+#pragma unroll
+            for (int m = 0; m < L; ++m) vsave[m] = x[m];
+        } else {
+            col_stash<T, L>(s_v, x, p, lane, TX);
+        }
         delta_residual<T, L>(x, vl, vr, s, act ? n : 0, a.k[d], a.k[a.ns + d], a.beta[d]);
         __syncthreads();  // coefficient staging complete
         chunk_solve<T, L>(x, act, p, P, lane, TX, s_inv, s_cp, a.k[d], s_z, s_g);
-        col_unstash_add<T, L>(s_v, x, p, lane, TX);
+        if constexpr (col_stash_in_regs(L)) {
+#pragma unroll
+            for (int m = 0; m < L; ++m) x[m] = vsave[m] + x[m];
+        } else {
+            col_unstash_add<T, L>(s_v, x, p, lane, TX);
+        }
     } else {
         __syncthreads();  // coefficient staging complete
         chunk_solve<T, L>(x, act, p, P, lane, TX, s_inv, s_cp, a.k[d], s_z, s_g);
