@@ -171,7 +171,6 @@ __global__ __launch_bounds__(kb_col_max_threads(L * sizeof(T))) void kb_sweep_co
     T* s_g = s_z + blockDim.x;
     int* s_item = reinterpret_cast<int*>(s_g + PMAX);
     T* s_dv = reinterpret_cast<T*>(s_item + 4);  // the item substrate's 7 Dirichlet slot values
-    T* s_v = s_dv + 8;                           // FP32: blockDim.x * L values (stash of v)
     const int t = threadIdx.x;
     const int n_items = *nitems;
     if (t == 0) s_item[0] = atomicAdd(ticket, 1);
@@ -248,12 +247,16 @@ __global__ __launch_bounds__(kb_col_max_threads(L * sizeof(T))) void kb_sweep_co
                 fetch(act_n, base_n);
             }
             if (DEV) {
-                // the values wait in LDS for x = v + delta (each thread its own entries)
-                col_stash<T, L>(s_v, x, p, lane, W);
+                // the values wait in registers for x = v + delta (no extra VGPRs at
+                // L <= 32: the chunk's values are live across the solve anyway)
+                T vsave[L];
+#pragma unroll
+                for (int m = 0; m < L; ++m) vsave[m] = x[m];
                 delta_residual<T, L>(x, hl, hr, s, act ? n : 0, kd, lam, bt);
                 __syncthreads();  // staging done / the previous tile's fold readers are done
                 chunk_solve<T, L>(x, act, p, P, lane, W, s_inv, s_cp, kd, s_z, s_g);
-                col_unstash_add<T, L>(s_v, x, p, lane, W);
+#pragma unroll
+                for (int m = 0; m < L; ++m) x[m] = vsave[m] + x[m];
             } else {
                 __syncthreads();  // staging done / the previous tile's fold readers are done
                 chunk_solve<T, L>(x, act, p, P, lane, W, s_inv, s_cp, kd, s_z, s_g);
