@@ -87,6 +87,7 @@ def make_env(pkg, w, n, ns, precision, tiles=None):
         env.set_tiles(tiles["Lx"], tiles["Ly"], tiles["Lz"], tiles["rows_x"], tiles["cols_y"],
                       tiles["cols_z"])
         env.set_x_grid(tiles.get("x_grid", 0))
+        env.set_col_runs(tiles.get("runs_y", 1), tiles.get("runs_z", 1))
         env.set_slab(tiles.get("slab", 0))
         env.set_substrate_streams(tiles.get("substrate_streams", 1))
     env.set_coefficients(w["D"], w["decay"])
@@ -145,11 +146,12 @@ def run_host_boundary(env, dt, updates, prec, reps=3, fix_basis=False):
             "pcie_bytes_down": int(host.size * (4 if fp32 else 8)), "reps": reps}
 
 
-def load_traffic(kernel_key, tiles):
+def load_traffic(kernel_key, tiles, runs=(1, 1)):
     """HBM bytes per launch of a sweep kernel, from the counter runs recorded
     in profiles/traffic.json, or None unless an entry was measured with this
-    exact kernel shape (tiles = Lx, Ly, Lz, rows_x, cols_y, cols_z): another
-    shape moves other bytes, so its figure is not reported."""
+    exact kernel shape (tiles = Lx, Ly, Lz, rows_x, cols_y, cols_z) and tile
+    order (runs = y and z tile sub-runs per XCD): another shape moves other
+    bytes, so its figure is not reported."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
@@ -159,17 +161,20 @@ def load_traffic(kernel_key, tiles):
         return None
     if not ent or [int(t) for t in ent.get("tiles", [])] != [int(t) for t in list(tiles)[:6]]:
         return None
+    if [int(r) for r in ent.get("runs", [1, 1])] != [int(r) for r in runs]:
+        return None
     return ent.get("bytes")
 
 
 def parse_tiles(spec):
-    """Lx,Ly,Lz,rows_x,cols_y,cols_z[,slab[,streams[,x_grid]]] -> shape dict."""
+    """Lx,Ly,Lz,rows_x,cols_y,cols_z[,slab[,streams[,x_grid[,runs_y,runs_z]]]] -> shape dict."""
     if not spec:
         return None
     v = [int(x) for x in spec.split(",")]
     return dict(Lx=v[0], Ly=v[1], Lz=v[2], rows_x=v[3], cols_y=v[4], cols_z=v[5],
                 slab=v[6] if len(v) > 6 else 0, substrate_streams=v[7] if len(v) > 7 else 1,
-                x_grid=v[8] if len(v) > 8 else 0)
+                x_grid=v[8] if len(v) > 8 else 0, runs_y=v[9] if len(v) > 9 else 1,
+                runs_z=v[10] if len(v) > 10 else 1)
 
 
 def host_cpu():
@@ -513,6 +518,7 @@ def run_large(pkg, n=800, steps=20, seed=3, tune=True, fixed=None):
         if tiles is not None:
             e.set_tiles(tiles["Lx"], tiles["Ly"], tiles["Lz"], tiles["rows_x"], tiles["cols_y"], tiles["cols_z"])
             e.set_x_grid(tiles.get("x_grid", 0))
+            e.set_col_runs(tiles.get("runs_y", 1), tiles.get("runs_z", 1))
             e.set_slab(tiles.get("slab", 0))
             e.set_substrate_streams(tiles.get("substrate_streams", 1))
         e.set_coefficients(D, decay)
@@ -537,9 +543,11 @@ def run_large(pkg, n=800, steps=20, seed=3, tune=True, fixed=None):
                      "roofline": {"bound": "hbm", "kernel": f"sweep_{names[dom]}", "alg_bytes_per_launch": alg,
                                   "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                                   "frac": achieved / PEAK_HBM_GBS,
-                                  "traffic": load_traffic(f"sweep_{names[dom]}_{prec}_{n}", e.tiles())},
+                                  "traffic": load_traffic(f"sweep_{names[dom]}_{prec}_{n}", e.tiles(),
+                                                          e.col_runs())},
                      "tiles": list(e.tiles()), "autotuned": tune and (fixed or {}).get(prec) is None,
-                     "schedule": {k: (tiles or {}).get(k, 0) for k in ("slab", "substrate_streams", "x_grid")}}
+                     "schedule": {k: (tiles or {}).get(k, d) for k, d in
+                                  (("slab", 0), ("substrate_streams", 1), ("x_grid", 0), ("runs_y", 1), ("runs_z", 1))}}
         envs[prec] = e
     f32 = envs["fp32"].download("soa").reshape(ns, -1)
     f64 = envs["fp64"].download("soa").reshape(ns, -1)
@@ -570,7 +578,8 @@ def main():
     ap.add_argument("--mode", default="default", choices=["default", "single", "ensemble", "serve", "large"])
     ap.add_argument("--requests", type=int, default=200)
     ap.add_argument("--no-seq", action="store_true", help="ensemble mode: skip the sequential plain solves")
-    ap.add_argument("--tiles", default="", help="fixed FP32 shapes Lx,Ly,Lz,rows_x,cols_y,cols_z[,slab[,streams[,x_grid]]] "
+    ap.add_argument("--tiles", default="", help="fixed FP32 shapes "
+                    "Lx,Ly,Lz,rows_x,cols_y,cols_z[,slab[,streams[,x_grid[,runs_y,runs_z]]]] "
                     "instead of autotuning (profiling runs)")
     ap.add_argument("--tiles64", default="", help="fixed FP64 shapes, same fields as --tiles")
     ap.add_argument("--repeats", type=int, default=20, help="ensemble mode: timed batched solves")
@@ -618,6 +627,7 @@ def main():
     env32 = make_env(pkg, w, n, ns, "fp32", tiles["fp32"])
     wall32, ms32, gwall32 = time_gpu(env32, dt, args.steps, args.warmup)
     tiles32 = env32.tiles()
+    runs32 = env32.col_runs()
     res = {}
     if not args.no_fp64:
         env64 = make_env(pkg, w, n, ns, "fp64", tiles["fp64"])
@@ -631,7 +641,7 @@ def main():
         res["fp64"] = dict(value=updates * args.steps / wall64, ms_per_step=1e3 * wall64 / args.steps,
                            eager_ms_per_step=1e3 * gwall64 / args.steps,
                            kernel_ms=dict(x=ms64[0], y=ms64[1], z=ms64[2], agents=ms64[3]),
-                           tiles=list(env64.tiles()))
+                           tiles=list(env64.tiles()), col_runs=list(env64.col_runs()))
         res["fp32_vs_fp64"] = dict(max_rel_err_per_substrate=errs, mass_drift_per_substrate=drift,
                                    steps=args.warmup + 2 * args.steps,
                                    norm="max|f32 - f64| / max|f64| per substrate")
@@ -650,7 +660,7 @@ def main():
     sweeps = 2 if (slab32 and dom == 0) else 1
     alg_bytes = updates * 2 * 4 * sweeps  # one FP32 read + one write per voxel-substrate per sweep
     achieved = alg_bytes / (ms32[dom] * 1e-3) / 1e9
-    traffic = load_traffic(f"sweep_{names[dom]}_fp32_{n}", tiles32)
+    traffic = load_traffic(f"sweep_{names[dom]}_fp32_{n}", tiles32, runs32)
     out = {
         "metric": METRIC,
         "value": updates * args.steps / wall32,
@@ -679,6 +689,8 @@ def main():
                       cols_y=tiles32[4], cols_z=tiles32[5], slab=slab32,
                       substrate_streams=(tiles["fp32"] or {}).get("substrate_streams", 1),
                       x_grid=(tiles["fp32"] or {}).get("x_grid", 0),
+                      runs_y=(tiles["fp32"] or {}).get("runs_y", 1),
+                      runs_z=(tiles["fp32"] or {}).get("runs_z", 1),
                       autotuned=not args.no_tune and not args.tiles),
         "aux_kernels": aux,
         "host_boundary": dict(host_boundary, note="field in host memory (FP64, reference layout): "

@@ -296,6 +296,18 @@ int bfa_get_substrate_streams(const bfa_env *env);
 int bfa_set_x_grid(bfa_env *env, int blocks_per_cu);
 int bfa_get_x_grid(const bfa_env *env, int *requested, int *resident);
 
+/* Order in which the y and z sweeps take their column tiles: each of the
+ * GPU's 8 XCDs sweeps one contiguous share of the tiles, cut into runs_y
+ * (runs_z) sub-runs that it advances together, so that the blocks it holds at
+ * one time read from that many separate stretches of the field; 1 (the
+ * default) is one contiguous run.  The arithmetic per line is the same, so
+ * results are bit-identical for every value.  Range [1, 64]; the autotuner
+ * measures it (at 400^3 x 4 FP32 the z sweep is fastest with a few runs).
+ * No reference counterpart (a launch parameter of the y and z sweeps,
+ * BioFVM/BioFVM_solvers.cpp:427-777). */
+int bfa_set_col_runs(bfa_env *env, int runs_y, int runs_z);
+int bfa_get_col_runs(const bfa_env *env, int *runs_y, int *runs_z);
+
 /* ---- batches of independent microenvironments -------------------------------
  *
  * A batch advances many microenvironments of different shapes and substrate

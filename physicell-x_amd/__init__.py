@@ -122,6 +122,8 @@ def lib():
     L.bfa_get_substrate_streams.argtypes = [_vp]
     L.bfa_set_x_grid.argtypes = [_vp, ctypes.c_int]
     L.bfa_get_x_grid.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.bfa_set_col_runs.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+    L.bfa_get_col_runs.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.bfa_batch_set_fusion.argtypes = [_vp, ctypes.c_int]
     L.bfa_batch_set_col_threads.argtypes = [_vp, ctypes.c_int]
     L.bfa_batch_set_streams.argtypes = [_vp, ctypes.c_int]
@@ -379,6 +381,16 @@ class Microenvironment:
         req, res = ctypes.c_int(), ctypes.c_int()
         self._check(lib().bfa_get_x_grid(self._h, ctypes.byref(req), ctypes.byref(res)), "x_grid")
         return req.value, res.value
+
+    def set_col_runs(self, runs_y, runs_z):
+        """Tile sub-runs each XCD advances together in the y and z sweeps (1: one run)."""
+        self._check(lib().bfa_set_col_runs(self._h, int(runs_y), int(runs_z)), "set_col_runs")
+
+    def col_runs(self):
+        """(runs_y, runs_z)."""
+        ry, rz = ctypes.c_int(), ctypes.c_int()
+        self._check(lib().bfa_get_col_runs(self._h, ctypes.byref(ry), ctypes.byref(rz)), "col_runs")
+        return ry.value, rz.value
 
     # -- field transfer -----------------------------------------------------------
     def upload(self, field, layout="ref"):

@@ -13,6 +13,8 @@ sweep kernels on the GPU (HIP events around each launch, via
   in {4, 8, 16, 25, 32}, with TX * ceil(n/L) <= 1024 threads, chosen per axis
   (25-node chunks cut a 400-node line into 16 chunks: 32 columns then fill
   eight whole wavefronts with no padded nodes);
+* y and z sweeps at the chosen shapes: how many sub-runs of tiles each XCD
+  advances together (``Microenvironment.set_col_runs``), per axis;
 * schedule: whole-volume sweeps, the two substrate halves swept on parallel
   streams, or x/y sweeps in k-slabs of a few planes (chosen by graph-replayed
   whole-step time).
@@ -33,6 +35,7 @@ COL_TX = (16, 32, 64)
 COL_L = (4, 8, 16, 25, 32)
 SLAB_BYTES = (8 << 20, 16 << 20, 32 << 20, 64 << 20, 128 << 20)
 X_GRID = (2, 3, 4, 5, 6, 8)
+COL_RUNS = (1, 2, 3, 4, 5, 6, 8, 12)
 
 _cache: dict = {}
 
@@ -65,8 +68,8 @@ def _step_ms(env, dt, reps):
 
 def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=6, cache_path=None,
              verbose=False):
-    """Return {"Lx", "rows_x", "Ly", "cols_y", "Lz", "cols_z", "x_grid", "slab",
-    "substrate_streams", "ms", "trials"}."""
+    """Return {"Lx", "rows_x", "Ly", "cols_y", "Lz", "cols_z", "x_grid", "runs_y",
+    "runs_z", "slab", "substrate_streams", "ms", "trials"}."""
     key = _key(nx, ny, nz, ns, precision)
     if key in _cache:
         return _cache[key]
@@ -131,6 +134,20 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=6, cache_path=
         shape = (best_x[1], best_y[1], best_z[1], best_x[2], best_y[2], best_z[2])
         env.set_tiles(*shape)
         best_grid = (best_x[0], best_x[3])
+        # tile sub-runs per XCD of the y and z sweeps, per axis (one launch
+        # times both axes at the same run count)
+        best_ry = (best_y[0], 1)
+        best_rz = (best_z[0], 1)
+        for R in COL_RUNS[1:]:
+            env.set_col_runs(R, R)
+            env.time_kernels(dt, 1)
+            ms = env.time_kernels(dt, reps)
+            trials.append({"axis": "yz", "runs": R, "ms_y": ms[1], "ms_z": ms[2]})
+            if ms[1] < best_ry[0]:
+                best_ry = (ms[1], R)
+            if ms[2] < best_rz[0]:
+                best_rz = (ms[2], R)
+        env.set_col_runs(best_ry[1], best_rz[1])
         # schedules, by graph-replayed step time: whole-volume sweeps, the two
         # substrate halves on parallel streams, x/y k-slabs
         cands = [(0, 1)]
@@ -150,9 +167,10 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=6, cache_path=
         env.set_slab(0)
         env.set_substrate_streams(1)
         res = {"Lx": best_x[1], "rows_x": best_x[2], "Ly": best_y[1], "cols_y": best_y[2],
-               "Lz": best_z[1], "cols_z": best_z[2], "x_grid": best_grid[1], "slab": best_sched[1],
+               "Lz": best_z[1], "cols_z": best_z[2], "x_grid": best_grid[1],
+               "runs_y": best_ry[1], "runs_z": best_rz[1], "slab": best_sched[1],
                "substrate_streams": best_sched[2],
-               "ms": {"x": best_grid[0], "y": best_y[0], "z": best_z[0], "step": best_sched[0]},
+               "ms": {"x": best_grid[0], "y": best_ry[0], "z": best_rz[0], "step": best_sched[0]},
                "trials": trials}
     finally:
         env.close()
@@ -176,6 +194,7 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=6, cache_path=
 def apply(env, res):
     env.set_tiles(res["Lx"], res["Ly"], res["Lz"], res["rows_x"], res["cols_y"], res["cols_z"])
     env.set_x_grid(res.get("x_grid", 0))
+    env.set_col_runs(res.get("runs_y", 1), res.get("runs_z", 1))
     env.set_slab(res.get("slab", 0))
     env.set_substrate_streams(res.get("substrate_streams", 1))
 

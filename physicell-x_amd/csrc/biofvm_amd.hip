@@ -116,6 +116,7 @@ struct bfa_env {
     bool keep_first_dt = false;  // reference caching: coefficients of the first dt kept for later dts
     int slab = 0;  // k-planes per x/y slab pair (0: whole-volume sweeps)
     int split = 1;  // substrate groups swept on parallel streams (1 or 2)
+    int runs[3] = {1, 1, 1};  // y / z sweeps: tile sub-runs per XCD (bfa_set_col_runs)
 
     hipStream_t aux[2] = {nullptr, nullptr};
     hipEvent_t ev_fork = nullptr;
@@ -335,6 +336,7 @@ bfa::SweepArgs<T> sweep_args(const bfa_env* e, int axis) {
     a.first = 0;
     a.count = axis == 0 ? (long long)e->ny * e->nz
                         : (axis == 1 ? (long long)e->nz * e->nx : (long long)e->ny * e->nx);
+    a.runs = e->runs[axis];
     return a;
 }
 
@@ -1761,6 +1763,23 @@ int bfa_get_x_grid(const bfa_env* e, int* requested, int* resident) {
     if (!e) return BFA_ERR_ARG;
     if (requested) *requested = e->xw_cap;
     if (resident) *resident = e->xw_occ;
+    return BFA_OK;
+}
+
+int bfa_set_col_runs(bfa_env* e, int runs_y, int runs_z) {
+    if (!e) return BFA_ERR_ARG;
+    if (runs_y < 1 || runs_y > 64 || runs_z < 1 || runs_z > 64)
+        return fail(e, BFA_ERR_ARG, "tile runs per XCD must be in [1, 64], got %d and %d", runs_y, runs_z);
+    e->runs[1] = runs_y;
+    e->runs[2] = runs_z;
+    invalidate_graph(e);
+    return BFA_OK;
+}
+
+int bfa_get_col_runs(const bfa_env* e, int* runs_y, int* runs_z) {
+    if (!e) return BFA_ERR_ARG;
+    if (runs_y) *runs_y = e->runs[1];
+    if (runs_z) *runs_z = e->runs[2];
     return BFA_OK;
 }
 

@@ -75,6 +75,7 @@ __device__ __forceinline__ SweepArgs<T> args_of(const EnvDesc<T>& e, int axis) {
     a.d0 = 0;
     a.first = 0;
     a.count = 0;
+    a.runs = 1;
     return a;
 }
 

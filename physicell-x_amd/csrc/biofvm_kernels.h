@@ -31,6 +31,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "biofvm_tilemap.h"
+
 namespace bfa {
 
 // Face selection for a boundary voxel: the last registered face (order xmin,
@@ -77,6 +79,9 @@ struct SweepArgs {
     // window first = k0*ny (x) or k0*nx (y), count = (k1-k0)*ny or (k1-k0)*nx.
     int d0;
     long long first, count;
+    // Column sweeps: sub-runs of tiles each XCD advances together (xcd_runs;
+    // 1: one contiguous run per XCD).  Unused by the x sweeps.
+    int runs;
 };
 
 // Offsets of the six face planes inside one substrate's shell.
@@ -444,17 +449,6 @@ template <typename T>
 __device__ __forceinline__ T shfl_up_t(T v, int d) { return __shfl_up(v, d, 64); }
 template <typename T>
 __device__ __forceinline__ T shfl_down_t(T v, int d) { return __shfl_down(v, d, 64); }
-
-// The workgroups of a launch are dispatched round-robin over the 8 XCDs
-// (linear workgroup id mod 8), each XCD with its own L2.  xcd_block maps the
-// linear id b of G workgroups to a logical tile index such that every XCD
-// runs one contiguous run of tiles, in order: neighbouring tiles that share
-// cache lines at their edges then meet in the same L2.  A bijection on [0, G).
-__device__ __forceinline__ int xcd_block(int b, int G) {
-    const int q = G >> 3, r = G & 7;
-    const int x = b & 7, i = b >> 3;
-    return x * q + (x < r ? x : r) + i;
-}
 
 // ---- wave-local x sweep: per-lane state and row routines ------------------
 // A row's P = ceil(nx/L) chunks sit in P consecutive lanes of a wavefront
@@ -891,7 +885,7 @@ __global__ __launch_bounds__(1024) void k_sweep_col(SweepArgs<T> a) {
     const int p = t / TX;
     // uniform over the block: keep the tile and substrate in scalar registers
     const int lb = __builtin_amdgcn_readfirstlane(
-        xcd_block(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y));
+        xcd_runs(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y, a.runs));
     const int d = __builtin_amdgcn_readfirstlane(a.d0 + lb / gridDim.x);
     stage_coefs<T>(s_inv, s_cp, a.inv + (long long)d * n, a.cp + (long long)d * n, n, P * L, t,
                    blockDim.x);

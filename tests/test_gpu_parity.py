@@ -495,6 +495,38 @@ def test_x_grid_is_bit_identical(pkg, precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_col_runs_are_bit_identical(pkg, precision):
+    """Any number of tile sub-runs per XCD in the y and z sweeps gives exactly
+    the default result (the order only changes which block solves which
+    column tile), on launches whose tile count is not a multiple of 8 and at
+    more runs than an XCD has tiles; out-of-range values are rejected and
+    leave the setting unchanged."""
+    p = make_problem(57, 33, 29, 3, seed=48, n_agents=120, dirichlet="mixed")
+    ref = gpu_env(pkg, p, precision)
+    try:
+        ref.set_tiles(8, 8, 8, 0, 16, 32)
+        ref.step(p.dt, 2)
+        r = ref.download()
+        assert ref.col_runs() == (1, 1)
+        for ry, rz in ((2, 3), (5, 1), (1, 7), (64, 64)):
+            env = gpu_env(pkg, p, precision)
+            try:
+                env.set_tiles(8, 8, 8, 0, 16, 32)
+                env.set_col_runs(ry, rz)
+                assert env.col_runs() == (ry, rz)
+                for bad in ((0, 1), (1, 0), (65, 1), (1, -2)):
+                    with pytest.raises(pkg.BioFVMError):
+                        env.set_col_runs(*bad)
+                assert env.col_runs() == (ry, rz)
+                env.step(p.dt, 2)
+                assert np.array_equal(env.download(), r), (ry, rz)
+            finally:
+                env.close()
+    finally:
+        ref.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("ns", [2, 3])
 def test_substrate_stream_schedule_is_bit_identical(pkg, precision, ns):
     """The two substrate halves swept on parallel streams give exactly the

@@ -258,7 +258,7 @@ def test_dt_caching_keeps_the_first_coefficients(pkg, precision, tol):
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_setters_right_after_asynchronous_steps(pkg, precision):
     """bfa_step only enqueues its graph launches.  Calls that drop the captured
-    graph (new coefficients, a new dt, new kernel shapes, new agents) may come
+    graph (new coefficients, a new dt, new kernel shapes or tile order, new agents) may come
     while those launches still run; the graph is drained before it is
     destroyed, so the result equals a run synchronised before every call.
     The batch's table regrowth (a member added after a step) likewise."""
@@ -270,6 +270,7 @@ def test_setters_right_after_asynchronous_steps(pkg, precision):
             calls = [lambda: env.set_coefficients(p.D * 1.5, p.decay),
                      lambda: env.step(p.dt * 2, 1),
                      lambda: env.set_tiles(8, 8, 8, 0, 16, 16),
+                     lambda: env.set_col_runs(3, 2),
                      lambda: env.set_agents(p.voxel[::-1].copy(), p.c1[::-1], p.c2[::-1], p.e2[::-1])]
             for call in calls:
                 env.step(p.dt, 15)
