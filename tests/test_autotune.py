@@ -1,7 +1,8 @@
-"""CPU tests of the batch autotuner's search: every admissible combination of
+"""CPU tests of the autotuners' searches: every admissible combination of
 chunk lengths and column-block size is timed once, rejected ones are
 skipped, and the fastest is left applied.  Driven through a stand-in batch
-whose step time is a known function of the shape (on a fake clock)."""
+or microenvironment whose times are a known function of the shape (on a
+fake clock)."""
 import importlib.util
 import os
 import sys
@@ -104,3 +105,112 @@ def test_batch_autotune_picks_the_fastest_admissible_shape(monkeypatch):
                 ok = 16 * P <= thr <= (512 if L * 4 >= 128 else 1024)
                 assert ((lx, L, thr) in tried) == ok, (lx, L, thr)
     assert res["trials"] == len(tried) + 3 * 4
+
+
+class FakeEnv:
+    """Stand-in Microenvironment of a 400-node cube: kernel times are a known
+    function of the shape, with unique minima at x chunk 8 with 4 blocks per
+    CU, y 25-node chunks x 32 columns in one tile run, z the same shape in
+    3 runs, and the two-substrate-group schedule 2 % faster."""
+
+    n = 400
+    log = []
+
+    def __init__(self, nx, ny, nz, ns, dx, precision):
+        self.shape = (8, 16, 16, 0, 32, 32)
+        self.grid = 0
+        self.runs = (1, 1)
+        self.slab = 0
+        self.split = 1
+        FakeEnv.log.append(self)
+        self.timed = []
+
+    def set_coefficients(self, D, decay):
+        pass
+
+    def set_dirichlet(self, *a):
+        pass
+
+    def upload(self, *a):
+        pass
+
+    def fill(self, *a):
+        pass
+
+    def set_tiles(self, lx, ly, lz, rx, cy, cz):
+        ly, lz = ly or 16, lz or 16  # 0: the library's default column shape
+        cy, cz = cy or 32, cz or 32
+        if rx == 0 and -(-self.n // lx) > 64:
+            raise Rejected("x chunk")
+        for L, TX in ((ly, cy), (lz, cz)):
+            if TX * -(-self.n // L) > 1024:
+                raise Rejected("column block")
+        self.shape = (lx, ly, lz, rx, cy, cz)
+
+    def set_x_grid(self, g):
+        self.grid = g
+
+    def x_grid(self):
+        return self.grid, 6 if self.shape[0] == 8 else 5
+
+    def set_col_runs(self, ry, rz):
+        self.runs = (ry, rz)
+
+    def set_slab(self, s):
+        self.slab = s
+
+    def set_substrate_streams(self, g):
+        self.split = g
+
+    def kernel_ms(self):
+        lx, ly, lz, rx, cy, cz = self.shape
+        x = 0.40 + (0.05 if rx else 0.0) + 0.01 * abs(lx - 8) / 4 + 0.002 * abs((self.grid or 6) - 4)
+        y = 0.37 + 0.003 * abs(ly - 25) + 0.0004 * abs(cy - 32) + 0.002 * (self.runs[0] - 1)
+        z = 0.42 + 0.003 * abs(lz - 25) + 0.0004 * abs(cz - 32) + 0.004 * abs(self.runs[1] - 3)
+        return [x, y, z, 0.03]
+
+    def time_kernels(self, dt, reps):
+        self.timed.append((self.shape, self.grid, self.runs))
+        return self.kernel_ms()
+
+    def step(self, dt, n):
+        t = sum(self.kernel_ms()) * (0.98 if self.split == 2 else 1.0) * (1.01 if self.slab else 1.0)
+        self.clock.t += t * n * 1e-3
+
+    def synchronize(self):
+        pass
+
+    def close(self):
+        pass
+
+
+def test_autotune_picks_the_fastest_shapes_grid_runs_and_schedule(monkeypatch):
+    at = load_autotune()
+    clock = Clock()
+    monkeypatch.setattr(time, "perf_counter", clock)
+    FakeEnv.clock = clock
+    FakeEnv.log = []
+
+    class Pkg:
+        Microenvironment = FakeEnv
+
+    res = at.autotune(Pkg, 400, 400, 400, 4, "fp32", 0.01)
+    assert (res["Lx"], res["rows_x"], res["x_grid"]) == (8, 0, 4)
+    assert (res["Ly"], res["cols_y"], res["Lz"], res["cols_z"]) == (25, 32, 25, 32)
+    assert (res["runs_y"], res["runs_z"]) == (1, 3)
+    assert (res["slab"], res["substrate_streams"]) == (0, 2)
+    assert res["ms"]["z"] == pytest.approx(0.42) and res["ms"]["y"] == pytest.approx(0.37)
+    env = FakeEnv.log[0]
+    timed = env.timed
+    # every admissible y/z shape was timed at one tile run, every run count at the chosen shape
+    yz = {(s[1], s[4]) for s, g, r in timed if s[0] == 8 and s[3] == 0 and r == (1, 1)}
+    for TX in at.COL_TX:
+        for L in at.COL_L:
+            ok = TX * -(-400 // L) <= 1024
+            assert ((L, TX) in yz) == ok, (L, TX)
+    runs = {r for s, g, r in timed if s == (8, 25, 25, 0, 32, 32)}
+    assert runs == {(R, R) for R in at.COL_RUNS}
+    # the result applies back onto an environment
+    e2 = FakeEnv(400, 400, 400, 4, 20.0, "fp32")
+    at.apply(e2, res)
+    assert e2.shape == (8, 25, 25, 0, 32, 32) and e2.grid == 4 and e2.runs == (1, 3) and e2.split == 2
