@@ -81,7 +81,10 @@ def make_workload(n, ns, n_agents, seed):
                 S=S, U=U, E=E, rs=rs, V=V)
 
 
-def make_env(pkg, w, n, ns, precision, tiles=None):
+def make_env(pkg, w, n, ns, precision, tiles=None, tune_agents=False):
+    """The workload's environment; tune_agents picks the agent step's stores
+    by measured step time (autotune.autotune_agents) and then restores the
+    initial field, else tiles["agent_lines"] (default whole lines) holds."""
     env = pkg.Microenvironment(n, n, n, ns, 20.0, precision)
     if tiles is not None:
         env.set_tiles(tiles["Lx"], tiles["Ly"], tiles["Lz"], tiles["rows_x"], tiles["cols_y"],
@@ -90,10 +93,15 @@ def make_env(pkg, w, n, ns, precision, tiles=None):
         env.set_col_runs(tiles.get("runs_y", 1), tiles.get("runs_z", 1))
         env.set_slab(tiles.get("slab", 0))
         env.set_substrate_streams(tiles.get("substrate_streams", 1))
+        env.set_agent_stores(bool(tiles.get("agent_lines", 1)))
     env.set_coefficients(w["D"], w["decay"])
     env.set_dirichlet(np.ones(ns, np.uint8), np.ones(6, np.uint8), np.tile(w["dval"], (6, 1)))
     env.upload(w["field"], "soa")
     env.set_agents(w["voxel"], w["c1"], w["c2"], w["e2"])
+    if tune_agents:
+        import importlib
+        importlib.import_module("physicellx_amd.autotune").autotune_agents(env, 0.01)
+        env.upload(w["field"], "soa")
     return env
 
 
@@ -167,14 +175,15 @@ def load_traffic(kernel_key, tiles, runs=(1, 1)):
 
 
 def parse_tiles(spec):
-    """Lx,Ly,Lz,rows_x,cols_y,cols_z[,slab[,streams[,x_grid[,runs_y,runs_z]]]] -> shape dict."""
+    """Lx,Ly,Lz,rows_x,cols_y,cols_z[,slab[,streams[,x_grid[,runs_y,runs_z[,agent_lines]]]]]
+    -> shape dict."""
     if not spec:
         return None
     v = [int(x) for x in spec.split(",")]
     return dict(Lx=v[0], Ly=v[1], Lz=v[2], rows_x=v[3], cols_y=v[4], cols_z=v[5],
                 slab=v[6] if len(v) > 6 else 0, substrate_streams=v[7] if len(v) > 7 else 1,
                 x_grid=v[8] if len(v) > 8 else 0, runs_y=v[9] if len(v) > 9 else 1,
-                runs_z=v[10] if len(v) > 10 else 1)
+                runs_z=v[10] if len(v) > 10 else 1, agent_lines=v[11] if len(v) > 11 else 1)
 
 
 def host_cpu():
@@ -521,10 +530,14 @@ def run_large(pkg, n=800, steps=20, seed=3, tune=True, fixed=None):
             e.set_col_runs(tiles.get("runs_y", 1), tiles.get("runs_z", 1))
             e.set_slab(tiles.get("slab", 0))
             e.set_substrate_streams(tiles.get("substrate_streams", 1))
+            e.set_agent_stores(bool(tiles.get("agent_lines", 1)))
         e.set_coefficients(D, decay)
         e.set_dirichlet(np.ones(ns, np.uint8), np.ones(6, np.uint8), np.tile(dval, (6, 1)))
         e.fill(dval)
         e.set_agents(voxel, c1, c2, e2)
+        if tune and (fixed or {}).get(prec) is None:
+            at.autotune_agents(e, 0.01)  # then back to the initial field
+            e.fill(dval)
         e.step(0.01, 1)
         e.synchronize()
         t0 = time.perf_counter()
@@ -547,7 +560,8 @@ def run_large(pkg, n=800, steps=20, seed=3, tune=True, fixed=None):
                                                           e.col_runs())},
                      "tiles": list(e.tiles()), "autotuned": tune and (fixed or {}).get(prec) is None,
                      "schedule": {k: (tiles or {}).get(k, d) for k, d in
-                                  (("slab", 0), ("substrate_streams", 1), ("x_grid", 0), ("runs_y", 1), ("runs_z", 1))}}
+                                  (("slab", 0), ("substrate_streams", 1), ("x_grid", 0), ("runs_y", 1), ("runs_z", 1))},
+                     "agent_stores": "whole lines" if e.agent_stores() else "single voxels"}
         envs[prec] = e
     f32 = envs["fp32"].download("soa").reshape(ns, -1)
     f64 = envs["fp64"].download("soa").reshape(ns, -1)
@@ -624,13 +638,15 @@ def main():
                     print(prec, json.dumps(t), file=sys.stderr)
     copy_gbs = 2 * 2 ** 30 / (pkg.copy_kernel_ms(2 ** 30, 10) * 1e-3) / 1e9
 
-    env32 = make_env(pkg, w, n, ns, "fp32", tiles["fp32"])
+    tune_ag = not args.no_tune
+    env32 = make_env(pkg, w, n, ns, "fp32", tiles["fp32"], tune_agents=tune_ag and not args.tiles)
     wall32, ms32, gwall32 = time_gpu(env32, dt, args.steps, args.warmup)
     tiles32 = env32.tiles()
     runs32 = env32.col_runs()
+    lines32 = env32.agent_stores()
     res = {}
     if not args.no_fp64:
-        env64 = make_env(pkg, w, n, ns, "fp64", tiles["fp64"])
+        env64 = make_env(pkg, w, n, ns, "fp64", tiles["fp64"], tune_agents=tune_ag and not args.tiles64)
         wall64, ms64, gwall64 = time_gpu(env64, dt, args.steps, args.warmup)
         # both have now run warmup + 2*steps steps from the same start
         f32 = env32.download("soa").reshape(ns, -1)
@@ -641,7 +657,8 @@ def main():
         res["fp64"] = dict(value=updates * args.steps / wall64, ms_per_step=1e3 * wall64 / args.steps,
                            eager_ms_per_step=1e3 * gwall64 / args.steps,
                            kernel_ms=dict(x=ms64[0], y=ms64[1], z=ms64[2], agents=ms64[3]),
-                           tiles=list(env64.tiles()), col_runs=list(env64.col_runs()))
+                           tiles=list(env64.tiles()), col_runs=list(env64.col_runs()),
+                           agent_stores="whole lines" if env64.agent_stores() else "single voxels")
         res["fp32_vs_fp64"] = dict(max_rel_err_per_substrate=errs, mass_drift_per_substrate=drift,
                                    steps=args.warmup + 2 * args.steps,
                                    norm="max|f32 - f64| / max|f64| per substrate")
@@ -691,6 +708,7 @@ def main():
                       x_grid=(tiles["fp32"] or {}).get("x_grid", 0),
                       runs_y=(tiles["fp32"] or {}).get("runs_y", 1),
                       runs_z=(tiles["fp32"] or {}).get("runs_z", 1),
+                      agent_stores="whole lines" if lines32 else "single voxels",
                       autotuned=not args.no_tune and not args.tiles),
         "aux_kernels": aux,
         "host_boundary": dict(host_boundary, note="field in host memory (FP64, reference layout): "

@@ -123,6 +123,8 @@ def lib():
     L.bfa_set_x_grid.argtypes = [_vp, ctypes.c_int]
     L.bfa_get_x_grid.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.bfa_set_col_runs.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+    L.bfa_set_agent_stores.argtypes = [_vp, ctypes.c_int]
+    L.bfa_get_agent_stores.argtypes = [_vp]
     L.bfa_get_col_runs.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.bfa_batch_set_fusion.argtypes = [_vp, ctypes.c_int]
     L.bfa_batch_set_col_threads.argtypes = [_vp, ctypes.c_int]
@@ -391,6 +393,14 @@ class Microenvironment:
         ry, rz = ctypes.c_int(), ctypes.c_int()
         self._check(lib().bfa_get_col_runs(self._h, ctypes.byref(ry), ctypes.byref(rz)), "col_runs")
         return ry.value, rz.value
+
+    def set_agent_stores(self, whole_lines):
+        """The agent step stores whole 128-byte lines (True, default) or single voxels."""
+        self._check(lib().bfa_set_agent_stores(self._h, 1 if whole_lines else 0), "set_agent_stores")
+
+    def agent_stores(self):
+        """True when the agent step stores whole lines."""
+        return lib().bfa_get_agent_stores(self._h) == 1
 
     # -- field transfer -----------------------------------------------------------
     def upload(self, field, layout="ref"):

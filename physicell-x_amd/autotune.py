@@ -191,6 +191,23 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=6, cache_path=
     return res
 
 
+def autotune_agents(env, dt=0.01, steps=20, reps=2):
+    """Choose how the agent step stores the field (whole 128-byte lines or
+    single voxels) for the environment's current agents and shapes, by
+    graph-replayed step time; the faster is left set.  Every trial advances
+    the field (2 * reps * (steps + 4) steps in all): tune before the run.
+    Returns {"whole_lines", "ms_step": {True: ms, False: ms}}."""
+    best = {}
+    for rep in range(reps):
+        for lines in (True, False):
+            env.set_agent_stores(lines)
+            t = _step_ms(env, dt, steps // 2)
+            best[lines] = min(best.get(lines, t), t)
+    pick = min(best, key=best.get)
+    env.set_agent_stores(pick)
+    return {"whole_lines": pick, "ms_step": best}
+
+
 def apply(env, res):
     env.set_tiles(res["Lx"], res["Ly"], res["Lz"], res["rows_x"], res["cols_y"], res["cols_z"])
     env.set_x_grid(res.get("x_grid", 0))

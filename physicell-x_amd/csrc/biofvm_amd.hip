@@ -99,6 +99,7 @@ struct bfa_env {
     int* ag_orig = nullptr;
     double* ag_e1 = nullptr;
     double* trk = nullptr;
+    bool agent_lines = true;  // agent step stores whole 128-byte lines (bfa_set_agent_stores)
 
     // graph of one step
     hipGraph_t graph = nullptr;
@@ -569,20 +570,23 @@ int launch_agents(bfa_env* e) {
     const long long work = (long long)e->nseg * e->ns;
     const unsigned blocks = (unsigned)((work + 255) / 256);
     const double W = e->dx * e->dx * e->dx;  // voxel volume
+#define BFA_AGENTS(T, LN)                                                                                     \
+    hipLaunchKernelGGL((bfa::k_agents<T, LN>), dim3(blocks), dim3(256), 0, e->stream, reinterpret_cast<T*>(e->u), \
+                       e->N, e->ns, e->nseg, e->nag, e->seg_off, e->seg_vox,                                     \
+                       reinterpret_cast<const double*>(e->agA), reinterpret_cast<const double*>(e->agB),         \
+                       reinterpret_cast<const double*>(e->agC), e->params, e->trk, e->ag_orig, e->ag_e1, W)
     if (e->prec == BFA_FP32) {
-        hipLaunchKernelGGL(bfa::k_agents_f32, dim3(blocks), dim3(256), 0, e->stream,
-                           reinterpret_cast<float*>(e->u), e->N, e->ns, e->nseg, e->nag,
-                           e->seg_off, e->seg_vox, reinterpret_cast<const double*>(e->agA),
-                           reinterpret_cast<const double*>(e->agB),
-                           reinterpret_cast<const double*>(e->agC), e->params, e->trk, e->ag_orig,
-                           e->ag_e1, W);
+        if (e->agent_lines)
+            BFA_AGENTS(float, true);
+        else
+            BFA_AGENTS(float, false);
     } else {
-        hipLaunchKernelGGL(bfa::k_agents_f64, dim3(blocks), dim3(256), 0, e->stream,
-                           reinterpret_cast<double*>(e->u), e->N, e->ns, e->nseg, e->nag,
-                           e->seg_off, e->seg_vox, reinterpret_cast<const double*>(e->agA),
-                           reinterpret_cast<const double*>(e->agB),
-                           reinterpret_cast<const double*>(e->agC), e->trk, e->ag_orig, e->ag_e1, W);
+        if (e->agent_lines)
+            BFA_AGENTS(double, true);
+        else
+            BFA_AGENTS(double, false);
     }
+#undef BFA_AGENTS
     LAUNCH_CHECK(e, "agents");
     return BFA_OK;
 }
@@ -1782,6 +1786,17 @@ int bfa_get_col_runs(const bfa_env* e, int* runs_y, int* runs_z) {
     if (runs_z) *runs_z = e->runs[2];
     return BFA_OK;
 }
+
+int bfa_set_agent_stores(bfa_env* e, int whole_lines) {
+    if (!e) return BFA_ERR_ARG;
+    if (whole_lines != 0 && whole_lines != 1)
+        return fail(e, BFA_ERR_ARG, "agent stores: 1 (whole lines) or 0 (single voxels), got %d", whole_lines);
+    if ((whole_lines == 1) != e->agent_lines) invalidate_graph(e);
+    e->agent_lines = whole_lines == 1;
+    return BFA_OK;
+}
+
+int bfa_get_agent_stores(const bfa_env* e) { return e ? (e->agent_lines ? 1 : 0) : -1; }
 
 int bfa_copy_kernel(size_t bytes, int reps, double* ms) {
     if (!ms || reps < 1 || bytes < 16) return BFA_ERR_ARG;

@@ -63,7 +63,7 @@ __global__ void k_thomas(int n, int ns, const double* __restrict__ params, doubl
 // ---- agents: ordered per-voxel composition ----------------------------------
 // Agents are sorted by device voxel (stable in agent index) on the host, so
 // each segment [off[q], off[q+1]) holds the agents of one voxel in ascending
-// agent order.  One lane per (segment, substrate) walks its segment in order:
+// agent order.  Each segment's agents are applied in order (agent_segment):
 // the reference's serial loop order (BioFVM/BioFVM_microenvironment.cpp:
 // 1370-1384) with the per-agent update of BioFVM/BioFVM_basic_agent.cpp:
 // 536-552.  FP64: rho = (rho + c1)/c2 + e2 in the reference's operation
@@ -100,18 +100,14 @@ __device__ __forceinline__ void track_after(double* __restrict__ trk, const int*
     trk[o] = trk[o] - e1[o];
 }
 
-__global__ void k_agents_f64(double* __restrict__ u, long long N, int ns, int nseg,
-                             int nag, const int* __restrict__ off,
-                             const long long* __restrict__ vox, const double* __restrict__ c1,
-                             const double* __restrict__ c2, const double* __restrict__ e2,
-                             double* __restrict__ trk, const int* __restrict__ orig,
-                             const double* __restrict__ e1, double W) {
-    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= (long long)nseg * ns) return;
-    const int d = (int)(gid / nseg);
-    const int q = (int)(gid - (long long)d * nseg);
-    double* p = u + (long long)d * N + vox[q];
-    double rho = *p;
+// The agents of segment q (one voxel) applied to its density rho of
+// substrate d, in ascending agent order: rho = (rho + c1)/c2 + e2 per agent,
+// with the tracking updates around each when trk is set.
+__device__ __forceinline__ double agent_segment(double rho, int d, int q, const int* __restrict__ off,
+                                                const double* __restrict__ c1, const double* __restrict__ c2,
+                                                const double* __restrict__ e2, double* __restrict__ trk,
+                                                const int* __restrict__ orig, const double* __restrict__ e1,
+                                                int ns, int nag, double W) {
     const int a0 = off[q], a1 = off[q + 1];
     const size_t o = (size_t)d * nag;
     for (int a = a0; a < a1; ++a) {
@@ -121,32 +117,97 @@ __global__ void k_agents_f64(double* __restrict__ u, long long N, int ns, int ns
         rho += e2[o + a];
         if (trk) track_after(trk, orig, e1, ns, d, a);
     }
-    *p = rho;
+    return rho;
 }
 
-__global__ void k_agents_f32(float* __restrict__ u, long long N, int ns, int nseg, int nag,
-                             const int* __restrict__ off, const long long* __restrict__ vox,
-                             const double* __restrict__ c1, const double* __restrict__ c2,
-                             const double* __restrict__ e2, const double* __restrict__ params,
-                             double* __restrict__ trk, const int* __restrict__ orig,
-                             const double* __restrict__ e1, double W) {
+// One agent step, one lane per (segment, substrate).  Two ways to store:
+//  - LINES == false: each lane reads its voxel, applies its segment and
+//    stores the voxel (8 or 4 bytes);
+//  - LINES == true: the field is read and written in whole 128-byte lines.
+//    The lane of the first segment (in voxel order) of substrate d that falls
+//    in a line owns the line: it loads the line, applies every segment of the
+//    line in place and stores the whole line back; the lanes of the line's
+//    other segments exit.  A line shared with the neighbouring substrate
+//    (N*sizeof(T) not a multiple of 128 bytes) is stored voxel by voxel, so
+//    each lane writes only its own substrate; the field allocation's zeroed
+//    slack covers the last line.
+// Single-voxel stores leave lines partly written in the caches, and their
+// later write-back (a read-modify-write in memory) can slow the next sweeps
+// by more than the agent step takes (at 800^3 x 2 with 10^6 agents, FP64 x
+// 3.51 against 3.21 ms); whole lines cost more instructions in the agent
+// step.  Which is faster depends on the workload (bfa_set_agent_stores).
+// FP64 (params unused) or FP32 deviations lifted to FP64 with the basis
+// b, s, 1/s at params[2*ns + d], [3*ns + d], [4*ns + d].
+template <typename T, bool LINES>
+__global__ void k_agents(T* __restrict__ u, long long N, int ns, int nseg, int nag, const int* __restrict__ off,
+                         const long long* __restrict__ vox, const double* __restrict__ c1,
+                         const double* __restrict__ c2, const double* __restrict__ e2,
+                         const double* __restrict__ params, double* __restrict__ trk,
+                         const int* __restrict__ orig, const double* __restrict__ e1, double W) {
+    constexpr bool DEV = sizeof(T) == 4;
+    constexpr int E = 128 / (int)sizeof(T);  // elements per line
+    using VA = typename Vec16A<T>::type;
+    constexpr int VW = 16 / (int)sizeof(T);
     const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= (long long)nseg * ns) return;
     const int d = (int)(gid / nseg);
     const int q = (int)(gid - (long long)d * nseg);
-    const double b = params[2 * ns + d], s = params[3 * ns + d], rs = params[4 * ns + d];
-    float* p = u + (long long)d * N + vox[q];
-    double rho = fma((double)*p, s, b);
-    const int a0 = off[q], a1 = off[q + 1];
-    const size_t o = (size_t)d * nag;
-    for (int a = a0; a < a1; ++a) {
-        if (trk) track_before(trk, orig, ns, d, a, rho, c1[o + a], c2[o + a], W);
-        rho += c1[o + a];
-        rho /= c2[o + a];
-        rho += e2[o + a];
-        if (trk) track_after(trk, orig, e1, ns, d, a);
+    const long long lo = (long long)d * N;  // substrate d's elements: [lo, lo + N)
+    double b = 0.0, s = 1.0, rs = 1.0;
+    if (DEV) {
+        b = params[2 * ns + d];
+        s = params[3 * ns + d];
+        rs = params[4 * ns + d];
     }
-    *p = (float)((rho - b) * rs);
+    if (!LINES) {
+        T* p = u + lo + vox[q];
+        const double rho = agent_segment(DEV ? fma((double)*p, s, b) : (double)*p, d, q, off, c1, c2, e2, trk,
+                                         orig, e1, ns, nag, W);
+        *p = DEV ? (T)((rho - b) * rs) : (T)rho;
+        return;
+    }
+    const long long g = (lo + vox[q]) / E;  // the line of this segment's voxel
+    if (q > 0 && (lo + vox[q - 1]) / E == g) return;  // an earlier segment's lane owns it
+    T* lp = u + g * E;
+    T line[E];
+#pragma unroll
+    for (int i = 0; i < E / VW; ++i) {
+        const VA t = reinterpret_cast<const VA*>(lp)[i];
+#pragma unroll
+        for (int w = 0; w < VW; ++w) line[i * VW + w] = t[w];
+    }
+    int r = q;
+    for (; r < nseg; ++r) {
+        const long long v = lo + vox[r];
+        if (v / E != g) break;
+        const int m = (int)(v - g * E);
+        T val = T(0);
+#pragma unroll
+        for (int i = 0; i < E; ++i)
+            if (i == m) val = line[i];
+        const double rho = agent_segment(DEV ? fma((double)val, s, b) : (double)val, d, r, off, c1, c2, e2,
+                                         trk, orig, e1, ns, nag, W);
+        val = DEV ? (T)((rho - b) * rs) : (T)rho;
+#pragma unroll
+        for (int i = 0; i < E; ++i)
+            if (i == m) line[i] = val;
+    }
+    if (g * E >= lo && g * E + E <= lo + N) {
+#pragma unroll
+        for (int i = 0; i < E / VW; ++i) {
+            VA t;
+#pragma unroll
+            for (int w = 0; w < VW; ++w) t[w] = line[i * VW + w];
+            reinterpret_cast<VA*>(lp)[i] = t;
+        }
+    } else {
+        for (int k = q; k < r; ++k) {
+            const int m = (int)(lo + vox[k] - g * E);
+#pragma unroll
+            for (int i = 0; i < E; ++i)
+                if (i == m) lp[i] = line[i];
+        }
+    }
 }
 
 // ---- standalone Dirichlet pass over the faces -------------------------------

@@ -214,3 +214,33 @@ def test_autotune_picks_the_fastest_shapes_grid_runs_and_schedule(monkeypatch):
     e2 = FakeEnv(400, 400, 400, 4, 20.0, "fp32")
     at.apply(e2, res)
     assert e2.shape == (8, 25, 25, 0, 32, 32) and e2.grid == 4 and e2.runs == (1, 3) and e2.split == 2
+
+
+class FakeAgentsEnv:
+    """Step time 1.00 ms with whole-line agent stores, 0.97 with single voxels."""
+
+    def __init__(self, clock):
+        self.clock = clock
+        self.lines = True
+        self.steps = 0
+
+    def set_agent_stores(self, lines):
+        self.lines = bool(lines)
+
+    def step(self, dt, n):
+        self.clock.t += (1.0 if self.lines else 0.97) * n * 1e-3
+        self.steps += n
+
+    def synchronize(self):
+        pass
+
+
+def test_autotune_agents_keeps_the_faster_store_form(monkeypatch):
+    at = load_autotune()
+    clock = Clock()
+    monkeypatch.setattr(time, "perf_counter", clock)
+    env = FakeAgentsEnv(clock)
+    res = at.autotune_agents(env, 0.01, steps=20, reps=2)
+    assert res["whole_lines"] is False and env.lines is False
+    assert res["ms_step"][True] == pytest.approx(1.0) and res["ms_step"][False] == pytest.approx(0.97)
+    assert env.steps == 2 * 2 * (20 + 4)
