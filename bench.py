@@ -84,7 +84,7 @@ def make_workload(n, ns, n_agents, seed):
 def make_env(pkg, w, n, ns, precision, tiles=None, tune_agents=False):
     """The workload's environment; tune_agents picks the agent step's stores
     by measured step time (autotune.autotune_agents) and then restores the
-    initial field, else tiles["agent_lines"] (default whole lines) holds."""
+    initial field, else tiles["agent_lines"] (default single voxels) holds."""
     env = pkg.Microenvironment(n, n, n, ns, 20.0, precision)
     if tiles is not None:
         env.set_tiles(tiles["Lx"], tiles["Ly"], tiles["Lz"], tiles["rows_x"], tiles["cols_y"],
@@ -93,7 +93,7 @@ def make_env(pkg, w, n, ns, precision, tiles=None, tune_agents=False):
         env.set_col_runs(tiles.get("runs_y", 1), tiles.get("runs_z", 1))
         env.set_slab(tiles.get("slab", 0))
         env.set_substrate_streams(tiles.get("substrate_streams", 1))
-        env.set_agent_stores(bool(tiles.get("agent_lines", 1)))
+        env.set_agent_stores(bool(tiles.get("agent_lines", 0)))
     env.set_coefficients(w["D"], w["decay"])
     env.set_dirichlet(np.ones(ns, np.uint8), np.ones(6, np.uint8), np.tile(w["dval"], (6, 1)))
     env.upload(w["field"], "soa")
@@ -183,7 +183,7 @@ def parse_tiles(spec):
     return dict(Lx=v[0], Ly=v[1], Lz=v[2], rows_x=v[3], cols_y=v[4], cols_z=v[5],
                 slab=v[6] if len(v) > 6 else 0, substrate_streams=v[7] if len(v) > 7 else 1,
                 x_grid=v[8] if len(v) > 8 else 0, runs_y=v[9] if len(v) > 9 else 1,
-                runs_z=v[10] if len(v) > 10 else 1, agent_lines=v[11] if len(v) > 11 else 1)
+                runs_z=v[10] if len(v) > 10 else 1, agent_lines=v[11] if len(v) > 11 else 0)
 
 
 def host_cpu():
@@ -530,7 +530,7 @@ def run_large(pkg, n=800, steps=20, seed=3, tune=True, fixed=None):
             e.set_col_runs(tiles.get("runs_y", 1), tiles.get("runs_z", 1))
             e.set_slab(tiles.get("slab", 0))
             e.set_substrate_streams(tiles.get("substrate_streams", 1))
-            e.set_agent_stores(bool(tiles.get("agent_lines", 1)))
+            e.set_agent_stores(bool(tiles.get("agent_lines", 0)))
         e.set_coefficients(D, decay)
         e.set_dirichlet(np.ones(ns, np.uint8), np.ones(6, np.uint8), np.tile(dval, (6, 1)))
         e.fill(dval)

@@ -309,17 +309,18 @@ int bfa_set_col_runs(bfa_env *env, int runs_y, int runs_z);
 int bfa_get_col_runs(const bfa_env *env, int *runs_y, int *runs_z);
 
 /* How the agent step (bfa_step, bfa_secretion_uptake) writes the field:
- * whole_lines = 1 (the default) loads and stores the whole 128-byte line
- * around each agent voxel (one lane per line, all the line's agent voxels
- * applied in place), 0 stores single voxels.  Results are bit-identical.
+ * whole_lines = 1 loads and stores the whole 128-byte line around each agent
+ * voxel (one lane per line, all the line's agent voxels applied in place),
+ * 0 (the default) stores single voxels.  Results are bit-identical.
  * Single-voxel stores leave lines partly written in the caches; their later
  * write-back costs a read-modify-write in memory that the next sweeps pay
  * for (measured: 800^3 x 2 with 10^6 agents, FP64 x sweep 3.51 ms after the
  * agent step against 3.21 ms; 400^3 x 4 FP64 with 10^5 agents, step 2.50
  * against 2.41 ms), while whole lines cost more instructions in the agent
- * step itself (400^3 x 4 FP32 with 10^5 agents: step 1.225 ms with single
- * voxels, 1.241 with whole lines).  autotune.autotune_agents times both on
- * the caller's agents.  No reference counterpart (the reference's agent loop
+ * step itself (400^3 x 4 FP32 with 10^5 agents: step 1.226 ms with single
+ * voxels, 1.238 with whole lines; the small members of a 256-member
+ * ensemble solved one by one: 1.11 against 1.02 s).  autotune.autotune_agents
+ * times both on the caller's agents.  No reference counterpart (the reference's agent loop
  * updates each voxel in place, BioFVM/BioFVM_basic_agent.cpp:536-552). */
 int bfa_set_agent_stores(bfa_env *env, int whole_lines);
 int bfa_get_agent_stores(const bfa_env *env);

@@ -395,7 +395,7 @@ class Microenvironment:
         return ry.value, rz.value
 
     def set_agent_stores(self, whole_lines):
-        """The agent step stores whole 128-byte lines (True, default) or single voxels."""
+        """The agent step stores whole 128-byte lines (True) or single voxels (False, default)."""
         self._check(lib().bfa_set_agent_stores(self._h, 1 if whole_lines else 0), "set_agent_stores")
 
     def agent_stores(self):

@@ -99,7 +99,7 @@ struct bfa_env {
     int* ag_orig = nullptr;
     double* ag_e1 = nullptr;
     double* trk = nullptr;
-    bool agent_lines = true;  // agent step stores whole 128-byte lines (bfa_set_agent_stores)
+    bool agent_lines = false;  // agent step stores whole 128-byte lines (bfa_set_agent_stores)
 
     // graph of one step
     hipGraph_t graph = nullptr;

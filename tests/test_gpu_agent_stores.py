@@ -1,5 +1,5 @@
 """GPU tests of the agent step's two store forms (bfa_set_agent_stores):
-whole 128-byte lines (the default) and single voxels give bit for bit the
+whole 128-byte lines and single voxels (the default) give bit for bit the
 same field and internalised totals, in FP64 and FP32, with agents piled onto
 shared voxels and lines, on a grid whose substrate blocks are not a whole
 number of lines (so the first and last lines of a substrate are shared with
@@ -60,10 +60,10 @@ def test_agent_store_setting_is_checked(pkg):
     p = make_problem(8, 8, 8, 1, seed=2201, n_agents=10)
     env = gpu_env(pkg, p, "fp64")
     try:
-        assert env.agent_stores()  # default: whole lines
-        env.set_agent_stores(False)
-        assert not env.agent_stores()
+        assert not env.agent_stores()  # default: single voxels
+        env.set_agent_stores(True)
+        assert env.agent_stores()
         assert pkg.lib().bfa_set_agent_stores(env._h, 2) != 0  # only 0 or 1
-        assert not env.agent_stores()
+        assert env.agent_stores()
     finally:
         env.close()
