@@ -100,6 +100,8 @@ struct bfa_env {
     double* ag_e1 = nullptr;
     double* trk = nullptr;
     bool agent_lines = false;  // agent step stores whole 128-byte lines (bfa_set_agent_stores)
+    int* long_seg = nullptr;   // segments of more than kLongSegment agents (k_agents_long)
+    int nlong = 0;
 
     // graph of one step
     hipGraph_t graph = nullptr;
@@ -588,6 +590,22 @@ int launch_agents(bfa_env* e) {
     }
 #undef BFA_AGENTS
     LAUNCH_CHECK(e, "agents");
+    if (e->nlong > 0) {
+        const unsigned lblocks = (unsigned)((64ll * e->nlong * e->ns + 255) / 256);
+        if (e->prec == BFA_FP32)
+            hipLaunchKernelGGL(bfa::k_agents_long<float>, dim3(lblocks), dim3(256), 0, e->stream,
+                               reinterpret_cast<float*>(e->u), e->N, e->ns, e->nlong, e->long_seg, e->nag,
+                               e->seg_off, e->seg_vox, reinterpret_cast<const double*>(e->agA),
+                               reinterpret_cast<const double*>(e->agB), reinterpret_cast<const double*>(e->agC),
+                               e->params, e->trk, e->ag_orig, e->ag_e1, W);
+        else
+            hipLaunchKernelGGL(bfa::k_agents_long<double>, dim3(lblocks), dim3(256), 0, e->stream,
+                               reinterpret_cast<double*>(e->u), e->N, e->ns, e->nlong, e->long_seg, e->nag,
+                               e->seg_off, e->seg_vox, reinterpret_cast<const double*>(e->agA),
+                               reinterpret_cast<const double*>(e->agB), reinterpret_cast<const double*>(e->agC),
+                               e->params, e->trk, e->ag_orig, e->ag_e1, W);
+        LAUNCH_CHECK(e, "agents (long segments)");
+    }
     return BFA_OK;
 }
 
@@ -939,6 +957,7 @@ void bfa_destroy(bfa_env* e) {
     dfree(e->ag_orig);
     dfree(e->ag_e1);
     dfree(e->trk);
+    dfree(e->long_seg);
     dfree(e->bulk_st);
     dfree(e->bulk_ups);
     for (int i = 0; i < 2; ++i) {
@@ -1306,6 +1325,32 @@ int bfa_download(bfa_env* e, double* field, int layout) {
     return cleanup(BFA_OK);
 }
 
+// Lists the segments of more than kLongSegment agents (k_agents_long applies
+// them), on the device, after every agent update.
+int build_long_segments(bfa_env* e) {
+    e->nlong = 0;
+    dfree(e->long_seg);
+    if (e->nseg == 0) return BFA_OK;
+    int* cnt = nullptr;
+    int rc;
+    if ((rc = dalloc(e, &e->long_seg, (size_t)e->nseg * sizeof(int))) || (rc = dalloc(e, &cnt, sizeof(int))))
+        return rc;
+    int h = 0;
+    hipError_t st = hipMemsetAsync(cnt, 0, sizeof(int), e->stream);
+    if (st == hipSuccess) {
+        hipLaunchKernelGGL(bfa::k_long_segments, dim3((unsigned)((e->nseg + 255) / 256)), dim3(256), 0, e->stream,
+                           e->nseg, e->seg_off, e->long_seg, cnt);
+        st = hipGetLastError();
+    }
+    if (st == hipSuccess) st = hipMemcpyAsync(&h, cnt, sizeof(int), hipMemcpyDeviceToHost, e->stream);
+    if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+    dfree(cnt);
+    if (st != hipSuccess) return fail(e, BFA_ERR_HIP, "long agent segments: %s", hipGetErrorString(st));
+    e->nlong = h;
+    if (h == 0) dfree(e->long_seg);
+    return BFA_OK;
+}
+
 int bfa_set_agents(bfa_env* e, int n, const int32_t* voxel, const uint8_t* active, const double* c1,
                    const double* c2, const double* e2) {
     if (!e) return BFA_ERR_ARG;
@@ -1318,6 +1363,8 @@ int bfa_set_agents(bfa_env* e, int n, const int32_t* voxel, const uint8_t* activ
     dfree(e->agB);
     dfree(e->agC);
     dfree(e->ag_orig);
+    dfree(e->long_seg);
+    e->nlong = 0;
     drop_tracking_if_resized(e, n);
     // host constants carry no e1: tracking kept on for the same agents keeps
     // the e1 it had, else bfa_track_internalized sets it
@@ -1389,7 +1436,7 @@ int bfa_set_agents(bfa_env* e, int n, const int32_t* voxel, const uint8_t* activ
     HIP_TRY(e, hipMemcpy(e->ag_orig, orig.data(), (size_t)m * sizeof(int), hipMemcpyHostToDevice));
     e->nag = m;
     e->nseg = nseg;
-    return BFA_OK;
+    return build_long_segments(e);
 }
 
 int bfa_track_internalized(bfa_env* e, int enable, const double* initial, const double* e1) {

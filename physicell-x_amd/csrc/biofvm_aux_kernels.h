@@ -138,6 +138,10 @@ __device__ __forceinline__ double agent_segment(double rho, int d, int q, const 
 // step.  Which is faster depends on the workload (bfa_set_agent_stores).
 // FP64 (params unused) or FP32 deviations lifted to FP64 with the basis
 // b, s, 1/s at params[2*ns + d], [3*ns + d], [4*ns + d].
+// Segments of more agents than this are applied by k_agents_long, a
+// wavefront per segment; k_agents leaves them alone.
+constexpr int kLongSegment = 64;
+
 template <typename T, bool LINES>
 __global__ void k_agents(T* __restrict__ u, long long N, int ns, int nseg, int nag, const int* __restrict__ off,
                          const long long* __restrict__ vox, const double* __restrict__ c1,
@@ -160,6 +164,7 @@ __global__ void k_agents(T* __restrict__ u, long long N, int ns, int nseg, int n
         rs = params[4 * ns + d];
     }
     if (!LINES) {
+        if (off[q + 1] - off[q] > kLongSegment) return;
         T* p = u + lo + vox[q];
         const double rho = agent_segment(DEV ? fma((double)*p, s, b) : (double)*p, d, q, off, c1, c2, e2, trk,
                                          orig, e1, ns, nag, W);
@@ -180,6 +185,7 @@ __global__ void k_agents(T* __restrict__ u, long long N, int ns, int nseg, int n
     for (; r < nseg; ++r) {
         const long long v = lo + vox[r];
         if (v / E != g) break;
+        if (off[r + 1] - off[r] > kLongSegment) continue;
         const int m = (int)(v - g * E);
         T val = T(0);
 #pragma unroll
@@ -208,6 +214,83 @@ __global__ void k_agents(T* __restrict__ u, long long N, int ns, int nseg, int n
                 if (i == m) lp[i] = line[i];
         }
     }
+}
+
+// Segments of more than kLongSegment agents (many agents in one voxel), one
+// wavefront per (segment, substrate), after k_agents: a segmented reduction
+// of the agents' affine maps rho -> rho/c2 + (c1/c2 + e2) over the lanes.
+// Lane l takes K = ceil(len/64) consecutive agents and composes their maps;
+// an inclusive Hillis-Steele scan of the composed maps over the 64 lanes
+// (composition g after f: (A_g A_f, A_g B_f + B_g)) gives each lane the map
+// of all agents before its own, so each lane starts its agents from the
+// density they see and walks them with the reference's operations
+// (rho += c1; rho /= c2; rho += e2, with the tracking updates), and the lane
+// holding the segment's last agent stores the voxel.  The lanes' starting
+// densities come from composed maps, so the result agrees with the serial
+// walk to rounding (a few ulps per lane boundary), not bit for bit.  long_seg
+// lists the long segments.  Launch: 64 * nlong * ns threads.
+template <typename T>
+__global__ __launch_bounds__(256) void k_agents_long(T* __restrict__ u, long long N, int ns, int nlong,
+                                                     const int* __restrict__ long_seg, int nag,
+                                                     const int* __restrict__ off, const long long* __restrict__ vox,
+                                                     const double* __restrict__ c1, const double* __restrict__ c2,
+                                                     const double* __restrict__ e2,
+                                                     const double* __restrict__ params, double* __restrict__ trk,
+                                                     const int* __restrict__ orig, const double* __restrict__ e1,
+                                                     double W) {
+    constexpr bool DEV = sizeof(T) == 4;
+    const long long wv = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wv >= (long long)nlong * ns) return;  // whole wavefronts
+    const int d = (int)(wv / nlong);
+    const int q = long_seg[wv - (long long)d * nlong];
+    const int a0 = off[q], a1 = off[q + 1];
+    const int K = (a1 - a0 + 63) / 64;
+    const int b0 = min(a1, a0 + lane * K), b1 = min(a1, b0 + K);
+    const size_t o = (size_t)d * nag;
+    double A = 1.0, B = 0.0;  // this lane's agents composed: rho -> A*rho + B
+    for (int a = b0; a < b1; ++a) {
+        const double ai = 1.0 / c2[o + a];
+        const double bi = c1[o + a] / c2[o + a] + e2[o + a];
+        B = fma(ai, B, bi);
+        A = ai * A;
+    }
+#pragma unroll
+    for (int sh = 1; sh < 64; sh <<= 1) {
+        const double Ap = __shfl_up(A, sh, 64), Bp = __shfl_up(B, sh, 64);
+        if (lane >= sh) {
+            B = fma(A, Bp, B);
+            A = A * Ap;
+        }
+    }
+    // the map of the agents before this lane's: the previous lane's inclusive map
+    const double Ae = __shfl_up(A, 1, 64), Be = __shfl_up(B, 1, 64);
+    T* p = u + (long long)d * N + vox[q];
+    double b = 0.0, s = 1.0, rs = 1.0;
+    if (DEV) {
+        b = params[2 * ns + d];
+        s = params[3 * ns + d];
+        rs = params[4 * ns + d];
+    }
+    const double rho0 = DEV ? fma((double)*p, s, b) : (double)*p;
+    double rho = lane == 0 ? rho0 : fma(Ae, rho0, Be);
+    for (int a = b0; a < b1; ++a) {
+        if (trk) track_before(trk, orig, ns, d, a, rho, c1[o + a], c2[o + a], W);
+        rho += c1[o + a];
+        rho /= c2[o + a];
+        rho += e2[o + a];
+        if (trk) track_after(trk, orig, e1, ns, d, a);
+    }
+    if (b1 == a1 && b0 < b1) *p = DEV ? (T)((rho - b) * rs) : (T)rho;
+}
+
+// Appends the index of every segment of more than kLongSegment agents to
+// list (in any order) and counts them.
+__global__ void k_long_segments(int nseg, const int* __restrict__ off, int* __restrict__ list,
+                                int* __restrict__ count) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nseg) return;
+    if (off[q + 1] - off[q] > kLongSegment) list[atomicAdd(count, 1)] = q;
 }
 
 // ---- standalone Dirichlet pass over the faces -------------------------------

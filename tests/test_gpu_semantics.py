@@ -4,7 +4,7 @@
   nodes overridden on faces and edges), the sweeps and the standalone pass;
 * internalised-substrate tracking per agent
   (track_internalized_substrates_in_each_agent), from host constants and from
-  raw agent state, with agents sharing voxels;
+  raw agent state, with agents sharing voxels, up to 2000 in one voxel;
 * the LOD_3D_AVX256D variant (no Dirichlet pass after the z sweep).
 
 Bars as in the other parity tests: FP64 1e-12, FP32 1e-5 (relative max-norm).
@@ -304,3 +304,48 @@ def test_setters_right_after_asynchronous_steps(pkg, precision):
 
     for a, c in zip(run_batch(False), run_batch(True)):
         assert np.array_equal(a, c)
+
+
+def crowded_problem(seed):
+    """Agents piled onto single voxels: 2000, 300, 65 and 64 agents (the last
+    one at the serial walk's limit), interleaved with scattered agents."""
+    p = make_problem(20, 18, 22, 2, seed=seed, n_agents=3000, dirichlet="all", export=15.0, collide=6)
+    rng = np.random.default_rng(seed)
+    hot = rng.choice(p.nx * p.ny * p.nz, 4, replace=False).astype(np.int32)
+    idx = rng.permutation(3000)
+    start = 0
+    for h, cnt in zip(hot, (2000, 300, 65, 64)):
+        p.voxel[np.sort(idx[start:start + cnt])] = h
+        start += cnt
+    p.agent_active[:] = 1
+    p.agent_active[idx[5]] = 0  # one inactive agent inside the 2000
+    return p
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("fp32", FP32_TOL)])
+@pytest.mark.parametrize("lines", [False, True])
+def test_crowded_voxels_match_oracle(pkg, precision, tol, lines):
+    """Segments of more than 64 agents go through the wavefront segmented
+    reduction (k_agents_long); field and internalised totals stay within the
+    bars of the oracle's serial walk, with either agent store form."""
+    p = crowded_problem(45)
+    orc = oracle_for(p)
+    ag = agents_for(p)
+    rng = np.random.default_rng(46)
+    start = rng.uniform(0, 100, (p.voxel.size, p.ns))
+    internal = start.copy()
+    env = gpu_env(pkg, p, precision)
+    try:
+        env.set_agent_stores(lines)
+        env.track_internalized(start, p.e1)
+        u = p.u0.copy()
+        for s in range(3):
+            orc.diffusion_step(u, p.dt)
+            orc.secretion_uptake(u, ag, internal)
+            env.step(p.dt, 1)
+            assert relmax(env.download(), u) <= tol, f"step {s}"
+            assert relmax(env.internalized(), internal) <= tol, f"step {s}"
+        # the crowded voxels moved: uptake by 2000 agents in one voxel
+        assert np.max(np.abs(internal - start)) > 1.0
+    finally:
+        env.close()
