@@ -1326,15 +1326,19 @@ int bfa_download(bfa_env* e, double* field, int layout) {
 }
 
 // Lists the segments of more than kLongSegment agents (k_agents_long applies
-// them), on the device, after every agent update.
+// them), on the device, after every agent update.  The per-segment kernel
+// skips those segments, so on failure the agents are dropped (none rather
+// than some applied) and the error returned.
 int build_long_segments(bfa_env* e) {
     e->nlong = 0;
     dfree(e->long_seg);
     if (e->nseg == 0) return BFA_OK;
     int* cnt = nullptr;
     int rc;
-    if ((rc = dalloc(e, &e->long_seg, (size_t)e->nseg * sizeof(int))) || (rc = dalloc(e, &cnt, sizeof(int))))
+    if ((rc = dalloc(e, &e->long_seg, (size_t)e->nseg * sizeof(int))) || (rc = dalloc(e, &cnt, sizeof(int)))) {
+        e->nseg = e->nag = 0;
         return rc;
+    }
     int h = 0;
     hipError_t st = hipMemsetAsync(cnt, 0, sizeof(int), e->stream);
     if (st == hipSuccess) {
@@ -1345,7 +1349,10 @@ int build_long_segments(bfa_env* e) {
     if (st == hipSuccess) st = hipMemcpyAsync(&h, cnt, sizeof(int), hipMemcpyDeviceToHost, e->stream);
     if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
     dfree(cnt);
-    if (st != hipSuccess) return fail(e, BFA_ERR_HIP, "long agent segments: %s", hipGetErrorString(st));
+    if (st != hipSuccess) {
+        e->nseg = e->nag = 0;
+        return fail(e, BFA_ERR_HIP, "long agent segments: %s", hipGetErrorString(st));
+    }
     e->nlong = h;
     if (h == 0) dfree(e->long_seg);
     return BFA_OK;
