@@ -355,6 +355,17 @@ int bfa_batch_download(bfa_batch *batch, int member_id, double *field, int layou
 int bfa_batch_set_agents(bfa_batch *batch, int member_id, int n, const int32_t *voxel,
                          const uint8_t *active, const double *c1, const double *c2,
                          const double *e2);
+/* Internalised-substrate tracking of a member's agents, as
+ * bfa_track_internalized / bfa_get_internalized for a single microenvironment
+ * (BioFVM/BioFVM_basic_agent.cpp:520-532,553-556): totals [n][ns] in the
+ * caller's agent order of the member's last bfa_batch_set_agents call
+ * (initial NULL: zeros), e1 [n][ns] the net export per step (NULL: kept from
+ * the previous call, else zeros).  A later bfa_batch_set_agents call with
+ * another agent count switches the member's tracking off.  Members that do
+ * not track are unaffected. */
+int bfa_batch_track_internalized(bfa_batch *batch, int member_id, int enable, const double *initial,
+                                 const double *e1);
+int bfa_batch_get_internalized(bfa_batch *batch, int member_id, double *out);
 /* Set every voxel of substrate d to values[d] (initial conditions without a
  * host transfer). */
 int bfa_batch_fill(bfa_batch *batch, int member_id, const double *values);

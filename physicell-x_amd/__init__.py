@@ -105,6 +105,8 @@ def lib():
     L.bfa_batch_upload.argtypes = [_vp, ctypes.c_int, _dp, ctypes.c_int]
     L.bfa_batch_download.argtypes = [_vp, ctypes.c_int, _dp, ctypes.c_int]
     L.bfa_batch_set_agents.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _i32, _u8, _dp, _dp, _dp]
+    L.bfa_batch_track_internalized.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
+    L.bfa_batch_get_internalized.argtypes = [_vp, ctypes.c_int, _dp]
     L.bfa_batch_set_member_steps.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
     L.bfa_batch_fill.argtypes = [_vp, ctypes.c_int, _dp]
     L.bfa_batch_step.argtypes = [_vp, ctypes.c_double, ctypes.c_int]
@@ -512,6 +514,7 @@ class Batch:
         self._h = h
         self.precision = _PREC[precision]
         self.members = {}
+        self._n_agents = {}  # agent count of each member's last set_agents call
 
     def close(self):
         if self._h is not None:
@@ -558,6 +561,7 @@ class Batch:
     def remove(self, mid):
         self._check(lib().bfa_batch_remove(self._h, int(mid)), "batch_remove")
         self.members.pop(mid, None)
+        self._n_agents.pop(mid, None)
 
     def upload(self, mid, field, layout="ref"):
         nx, ny, nz, ns = self.members[mid]
@@ -580,6 +584,27 @@ class Batch:
                                                None if act is None else act.ctypes.data_as(_u8),
                                                _d(_f64(c1, n * ns)), _d(_f64(c2, n * ns)),
                                                _d(_f64(e2, n * ns))), "batch_set_agents")
+        self._n_agents[mid] = n
+
+    def track_internalized(self, mid, initial=None, e1=None):
+        """Internalised-substrate tracking for member mid's current agents:
+        initial totals [n][ns] (None: zeros), net export e1 [n][ns] (None:
+        kept, else zeros)."""
+        cnt = self._n_agents.get(mid, 0) * self.members[mid][3]
+        ini = None if initial is None else _f64(initial, cnt)
+        ee = None if e1 is None else _f64(e1, cnt)
+        self._check(lib().bfa_batch_track_internalized(self._h, int(mid), 1, None if ini is None else _d(ini),
+                                                       None if ee is None else _d(ee)), "batch_track_internalized")
+
+    def stop_tracking_internalized(self, mid):
+        self._check(lib().bfa_batch_track_internalized(self._h, int(mid), 0, None, None),
+                    "batch_track_internalized")
+
+    def internalized(self, mid):
+        """Member mid's per-agent internalised totals [n][ns] (caller's agent order)."""
+        out = np.empty((self._n_agents.get(mid, 0), self.members[mid][3]))
+        self._check(lib().bfa_batch_get_internalized(self._h, int(mid), _d(out)), "batch_get_internalized")
+        return out
 
     def fill(self, mid, values):
         """Set every voxel of substrate d to values[d]."""

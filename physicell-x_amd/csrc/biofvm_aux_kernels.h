@@ -74,23 +74,12 @@ __global__ void k_thomas(int n, int ns, const double* __restrict__ params, doubl
 // Constants are stored [ns][nagents_sorted] so lanes of one substrate read
 // consecutive entries.
 // Internalised-substrate tracking of agent a (sorted position) before its
-// update of density rho (track_internalized_substrates_in_each_agent,
-// BioFVM/BioFVM_basic_agent.cpp:520-532): I -= W*((1 - c2)*rho + c1)/c2,
-// each operation rounded on its own as the reference's vector statements.
-// trk and e1 are [n][ns] in the caller's agent order; orig maps the sorted
-// position to that order.
+// update of density rho (track_uptake); trk and e1 are [n][ns] in the
+// caller's agent order; orig maps the sorted position to that order.
 __device__ __forceinline__ void track_before(double* __restrict__ trk, const int* __restrict__ orig,
                                              int ns, int d, int a, double rho, double c1, double c2,
                                              double W) {
-#pragma clang fp contract(off)
-    double t = 1.0;
-    t -= c2;
-    t *= rho;
-    t += c1;
-    t /= c2;
-    t *= W;
-    double* I = trk + (size_t)orig[a] * ns + d;
-    *I = *I - t;
+    track_uptake(trk + (size_t)orig[a] * ns + d, rho, c1, c2, W);
 }
 
 // ... and after it: I -= e1 (the net export, :553-556).

@@ -34,6 +34,7 @@ struct EnvDesc {
     const double* params;  // [5][ns]: D, decay, b, s, 1/s (FP32 deviation basis)
     int steps;  // steps of the current run this member takes; items of later steps are skipped
     int pad;
+    double W;   // voxel volume dx^3 (internalised-substrate tracking)
 };
 
 // x sweep item: rows [row0, row0 + nrows) of substrate d of member env.
@@ -471,21 +472,30 @@ struct AgentEntry {
     int env, d;
 };
 
+// Agent step of the batch: one lane per (segment, substrate) entry walks its
+// agents in order, as k_agents.  trk[a] (agent-entry order) is the address of
+// the agent's internalised total for the entry's substrate, or 0 when its
+// member does not track; e1[a] is the agent's net export for that substrate.
 template <typename T>
 __global__ void kb_agents_f64(const EnvDesc<T>* __restrict__ envs,
                               const AgentEntry* __restrict__ ent, const int* __restrict__ nent,
                               const double* __restrict__ c1, const double* __restrict__ c2,
-                              const double* __restrict__ e2, int step) {
+                              const double* __restrict__ e2, const unsigned long long* __restrict__ trk,
+                              const double* __restrict__ e1, int step) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= *nent) return;
     const AgentEntry en = ent[q];
-    if (step >= envs[en.env].steps) return;
+    const EnvDesc<T>& e = envs[en.env];
+    if (step >= e.steps) return;
     double* p = reinterpret_cast<double*>(en.addr);
     double rho = *p;
     for (int a = en.a0; a < en.a1; ++a) {
+        double* I = reinterpret_cast<double*>(trk[a]);
+        if (I) track_uptake(I, rho, c1[a], c2[a], e.W);
         rho += c1[a];
         rho /= c2[a];
         rho += e2[a];
+        if (I) *I = *I - e1[a];
     }
     *p = rho;
 }
@@ -494,7 +504,8 @@ template <typename T>
 __global__ void kb_agents_f32(const EnvDesc<T>* __restrict__ envs,
                               const AgentEntry* __restrict__ ent, const int* __restrict__ nent,
                               const double* __restrict__ c1, const double* __restrict__ c2,
-                              const double* __restrict__ e2, int step) {
+                              const double* __restrict__ e2, const unsigned long long* __restrict__ trk,
+                              const double* __restrict__ e1, int step) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= *nent) return;
     const AgentEntry en = ent[q];
@@ -505,9 +516,12 @@ __global__ void kb_agents_f32(const EnvDesc<T>* __restrict__ envs,
     float* p = reinterpret_cast<float*>(en.addr);
     double rho = fma((double)*p, s, b);
     for (int a = en.a0; a < en.a1; ++a) {
+        double* I = reinterpret_cast<double*>(trk[a]);
+        if (I) track_uptake(I, rho, c1[a], c2[a], e.W);
         rho += c1[a];
         rho /= c2[a];
         rho += e2[a];
+        if (I) *I = *I - e1[a];
     }
     *p = (float)((rho - b) * rs);
 }

@@ -114,6 +114,21 @@ __device__ __forceinline__ T dir_value(const SweepArgs<T>& a, int d, int i, int 
     return a.dval[dir_slot(i, j, k, a.nx, a.ny, a.nz, a.reg) * a.ns + d];
 }
 
+// Internalised-substrate tracking of one agent on its total I, before the
+// agent's update of density rho (track_internalized_substrates_in_each_agent,
+// BioFVM/BioFVM_basic_agent.cpp:520-532): I -= W*((1 - c2)*rho + c1)/c2,
+// each operation rounded on its own as the reference's vector statements.
+__device__ __forceinline__ void track_uptake(double* I, double rho, double c1, double c2, double W) {
+#pragma clang fp contract(off)
+    double t = 1.0;
+    t -= c2;
+    t *= rho;
+    t += c1;
+    t /= c2;
+    t *= W;
+    *I = *I - t;
+}
+
 // ---- chunk recurrences (positions s..s+L-1 of a padded line) ----------------
 // Coefficient arrays are padded to P*L entries with 1/den = c' = 0 past the
 // line end, and padded values are 0, so the recurrences need no masking: the

@@ -336,3 +336,49 @@ def test_served_jobs_equal_alone_and_oracle(pkg, precision):
             assert np.array_equal(alone.download(mid), j.result), (j.nx, j.ny, j.nz, j.ns)
         err = per_substrate_err(j.result, oracle_run(p, j.steps), p.ns)
         assert err <= TOL[precision], (j.nx, j.ny, j.nz, j.ns, err)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_internalized_tracking_equals_single_environment(pkg, precision):
+    """Members that track their agents' internalised substrates
+    (bfa_batch_track_internalized), beside members that do not, get the
+    totals of a single microenvironment and of the oracle within the bars
+    (the batch kernels' chunk lengths round differently from the plain
+    sweeps')."""
+    ps = problems(seed0=300)[:4]
+    steps = 3
+    rng = np.random.default_rng(301)
+    starts = [rng.uniform(0, 100, (p.voxel.size, p.ns)) for p in ps]
+    with pkg.Batch(precision) as b:
+        ids = [add(b, p) for p in ps]
+        for k in (0, 2):  # members 1 and 3 do not track
+            b.track_internalized(ids[k], starts[k], ps[k].e1)
+        b.step(ps[0].dt, steps)
+        b.synchronize()
+        for k in (0, 2):
+            p = ps[k]
+            env = gpu_env(pkg, p, precision)
+            try:
+                env.track_internalized(starts[k], p.e1)
+                env.step(p.dt, steps)
+                assert relmax(b.download(ids[k]), env.download()) <= TOL[precision]
+                assert relmax(b.internalized(ids[k]), env.internalized()) <= TOL[precision]
+            finally:
+                env.close()
+            orc = oracle_for(p)
+            ag = agents_for(p)
+            internal = starts[k].copy()
+            u = p.u0.copy()
+            for _ in range(steps):
+                orc.diffusion_step(u, p.dt)
+                orc.secretion_uptake(u, ag, internal)
+            assert relmax(b.internalized(ids[k]), internal) <= TOL[precision]
+        with pytest.raises(pkg.BioFVMError, match="tracking is off"):
+            b.internalized(ids[1])
+        # a new agent list of another length switches the member's tracking off
+        p = ps[0]
+        b.set_agents(ids[0], p.voxel[:-1], p.c1[:-1], p.c2[:-1], p.e2[:-1])
+        with pytest.raises(pkg.BioFVMError, match="tracking is off"):
+            b.internalized(ids[0])
+        b.step(p.dt, 1)
+        b.synchronize()
