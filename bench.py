@@ -559,8 +559,8 @@ def run_large(pkg, n=800, steps=20, seed=3, tune=True, fixed=None):
                                   "traffic": load_traffic(f"sweep_{names[dom]}_{prec}_{n}", e.tiles(),
                                                           e.col_runs())},
                      "tiles": list(e.tiles()), "autotuned": tune and (fixed or {}).get(prec) is None,
-                     "schedule": {k: (tiles or {}).get(k, d) for k, d in
-                                  (("slab", 0), ("substrate_streams", 1), ("x_grid", 0), ("runs_y", 1), ("runs_z", 1))},
+                     "schedule": {"slab": e.slab(), "substrate_streams": e.substrate_streams(),
+                                  "x_grid": e.x_grid()[0], "runs_y": e.col_runs()[0], "runs_z": e.col_runs()[1]},
                      "agent_stores": "whole lines" if e.agent_stores() else "single voxels"}
         envs[prec] = e
     f32 = envs["fp32"].download("soa").reshape(ns, -1)
@@ -644,6 +644,7 @@ def main():
     tiles32 = env32.tiles()
     runs32 = env32.col_runs()
     lines32 = env32.agent_stores()
+    split32 = env32.substrate_streams()
     res = {}
     if not args.no_fp64:
         env64 = make_env(pkg, w, n, ns, "fp64", tiles["fp64"], tune_agents=tune_ag and not args.tiles64)
@@ -658,6 +659,7 @@ def main():
                            eager_ms_per_step=1e3 * gwall64 / args.steps,
                            kernel_ms=dict(x=ms64[0], y=ms64[1], z=ms64[2], agents=ms64[3]),
                            tiles=list(env64.tiles()), col_runs=list(env64.col_runs()),
+                           substrate_streams=env64.substrate_streams(), slab=env64.slab(),
                            agent_stores="whole lines" if env64.agent_stores() else "single voxels")
         res["fp32_vs_fp64"] = dict(max_rel_err_per_substrate=errs, mass_drift_per_substrate=drift,
                                    steps=args.warmup + 2 * args.steps,
@@ -704,7 +706,7 @@ def main():
         "eager_ms_per_step": 1e3 * gwall32 / args.steps,
         "tiles": dict(Lx=tiles32[0], Ly=tiles32[1], Lz=tiles32[2], rows_x=tiles32[3],
                       cols_y=tiles32[4], cols_z=tiles32[5], slab=slab32,
-                      substrate_streams=(tiles["fp32"] or {}).get("substrate_streams", 1),
+                      substrate_streams=split32,
                       x_grid=(tiles["fp32"] or {}).get("x_grid", 0),
                       runs_y=(tiles["fp32"] or {}).get("runs_y", 1),
                       runs_z=(tiles["fp32"] or {}).get("runs_z", 1),

@@ -192,20 +192,27 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=6, cache_path=
 
 
 def autotune_agents(env, dt=0.01, steps=20, reps=2):
-    """Choose how the agent step stores the field (whole 128-byte lines or
-    single voxels) for the environment's current agents and shapes, by
-    graph-replayed step time; the faster is left set.  Every trial advances
-    the field (2 * reps * (steps + 4) steps in all): tune before the run.
-    Returns {"whole_lines", "ms_step": {True: ms, False: ms}}."""
+    """Choose, for the environment's current agents and shapes, how the agent
+    step stores the field (whole 128-byte lines or single voxels) and, with
+    several substrates and no k-slab schedule, whether the two substrate
+    halves run on parallel streams (each half's agent step then overlaps the
+    other half's sweeps), by graph-replayed step time; the fastest pair is
+    left set.  Every trial advances the field (reps * (steps + 4) steps per
+    candidate): tune before the run.  Returns {"whole_lines",
+    "substrate_streams", "ms_step": {(whole_lines, streams): ms}}."""
+    splits = (1, 2) if env.ns > 1 and env.slab() == 0 else (env.substrate_streams(),)
     best = {}
     for rep in range(reps):
         for lines in (True, False):
-            env.set_agent_stores(lines)
-            t = _step_ms(env, dt, steps // 2)
-            best[lines] = min(best.get(lines, t), t)
+            for split in splits:
+                env.set_agent_stores(lines)
+                env.set_substrate_streams(split)
+                t = _step_ms(env, dt, steps // 2)
+                best[(lines, split)] = min(best.get((lines, split), t), t)
     pick = min(best, key=best.get)
-    env.set_agent_stores(pick)
-    return {"whole_lines": pick, "ms_step": best}
+    env.set_agent_stores(pick[0])
+    env.set_substrate_streams(pick[1])
+    return {"whole_lines": pick[0], "substrate_streams": pick[1], "ms_step": best}
 
 
 def apply(env, res):

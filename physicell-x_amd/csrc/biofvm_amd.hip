@@ -550,7 +550,12 @@ int enqueue_xy_slabs(bfa_env* e) {
 // The three sweeps of two substrate groups on the two side streams, so that
 // each kernel's tail overlaps the other group's work.  Forks from and joins
 // back to e->stream.
-int enqueue_split(bfa_env* e) {
+int launch_agents_win(bfa_env* e, hipStream_t st, int d0, int nd);
+
+// The two substrate halves on the two side streams; with agents (a step)
+// each half's agent step follows its z sweep on its stream, so one half's
+// agents overlap the other half's sweeps.
+int enqueue_split(bfa_env* e, bool agents) {
     HIP_TRY(e, hipEventRecord(e->ev_fork, e->stream));
     const int h = (e->ns + 1) / 2;
     for (int g = 0; g < 2; ++g) {
@@ -561,20 +566,25 @@ int enqueue_split(bfa_env* e) {
             int rc = launch_sweep_win(e, a, st, Win{d0, nd, 0, axis_lines(e, a)});
             if (rc) return rc;
         }
+        if (agents) {
+            int rc = launch_agents_win(e, st, d0, nd);
+            if (rc) return rc;
+        }
         HIP_TRY(e, hipEventRecord(e->ev_join[g], st));
     }
     for (int g = 0; g < 2; ++g) HIP_TRY(e, hipStreamWaitEvent(e->stream, e->ev_join[g], 0));
     return BFA_OK;
 }
 
-int launch_agents(bfa_env* e) {
-    if (e->nseg == 0) return BFA_OK;
-    const long long work = (long long)e->nseg * e->ns;
+// The agent step over substrates [d0, d0 + nd) on stream st.
+int launch_agents_win(bfa_env* e, hipStream_t st, int d0, int nd) {
+    if (e->nseg == 0 || nd <= 0) return BFA_OK;
+    const long long work = (long long)e->nseg * nd;
     const unsigned blocks = (unsigned)((work + 255) / 256);
     const double W = e->dx * e->dx * e->dx;  // voxel volume
 #define BFA_AGENTS(T, LN)                                                                                     \
-    hipLaunchKernelGGL((bfa::k_agents<T, LN>), dim3(blocks), dim3(256), 0, e->stream, reinterpret_cast<T*>(e->u), \
-                       e->N, e->ns, e->nseg, e->nag, e->seg_off, e->seg_vox,                                     \
+    hipLaunchKernelGGL((bfa::k_agents<T, LN>), dim3(blocks), dim3(256), 0, st, reinterpret_cast<T*>(e->u), e->N, \
+                       e->ns, d0, nd, e->nseg, e->nag, e->seg_off, e->seg_vox,                                   \
                        reinterpret_cast<const double*>(e->agA), reinterpret_cast<const double*>(e->agB),         \
                        reinterpret_cast<const double*>(e->agC), e->params, e->trk, e->ag_orig, e->ag_e1, W)
     if (e->prec == BFA_FP32) {
@@ -591,23 +601,23 @@ int launch_agents(bfa_env* e) {
 #undef BFA_AGENTS
     LAUNCH_CHECK(e, "agents");
     if (e->nlong > 0) {
-        const unsigned lblocks = (unsigned)((64ll * e->nlong * e->ns + 255) / 256);
+        const unsigned lblocks = (unsigned)((64ll * e->nlong * nd + 255) / 256);
+#define BFA_AGENTS_LONG(T)                                                                                    \
+    hipLaunchKernelGGL(bfa::k_agents_long<T>, dim3(lblocks), dim3(256), 0, st, reinterpret_cast<T*>(e->u), e->N, \
+                       e->ns, d0, nd, e->nlong, e->long_seg, e->nag, e->seg_off, e->seg_vox,                    \
+                       reinterpret_cast<const double*>(e->agA), reinterpret_cast<const double*>(e->agB),         \
+                       reinterpret_cast<const double*>(e->agC), e->params, e->trk, e->ag_orig, e->ag_e1, W)
         if (e->prec == BFA_FP32)
-            hipLaunchKernelGGL(bfa::k_agents_long<float>, dim3(lblocks), dim3(256), 0, e->stream,
-                               reinterpret_cast<float*>(e->u), e->N, e->ns, e->nlong, e->long_seg, e->nag,
-                               e->seg_off, e->seg_vox, reinterpret_cast<const double*>(e->agA),
-                               reinterpret_cast<const double*>(e->agB), reinterpret_cast<const double*>(e->agC),
-                               e->params, e->trk, e->ag_orig, e->ag_e1, W);
+            BFA_AGENTS_LONG(float);
         else
-            hipLaunchKernelGGL(bfa::k_agents_long<double>, dim3(lblocks), dim3(256), 0, e->stream,
-                               reinterpret_cast<double*>(e->u), e->N, e->ns, e->nlong, e->long_seg, e->nag,
-                               e->seg_off, e->seg_vox, reinterpret_cast<const double*>(e->agA),
-                               reinterpret_cast<const double*>(e->agB), reinterpret_cast<const double*>(e->agC),
-                               e->params, e->trk, e->ag_orig, e->ag_e1, W);
+            BFA_AGENTS_LONG(double);
+#undef BFA_AGENTS_LONG
         LAUNCH_CHECK(e, "agents (long segments)");
     }
     return BFA_OK;
 }
+
+int launch_agents(bfa_env* e) { return launch_agents_win(e, e->stream, 0, e->ns); }
 
 // Substrates whose Dirichlet values are applied (dirichlet_activation_vector).
 unsigned long long face_active(const bfa_env* e) {
@@ -726,6 +736,8 @@ int launch_faces(bfa_env* e) {
     return BFA_OK;
 }
 
+bool split_schedule(const bfa_env* e) { return e->split > 1 && e->ns > 1 && e->slab == 0; }
+
 int enqueue_diffusion(bfa_env* e) {
     if (!e->hplanes.empty()) {
         // per-voxel Dirichlet values: the reference's separate face passes
@@ -738,7 +750,7 @@ int enqueue_diffusion(bfa_env* e) {
         }
         return (any && e->variant == BFA_LOD_3D_BLOCKING) ? launch_faces(e) : BFA_OK;
     }
-    if (e->split > 1 && e->ns > 1 && e->slab == 0) return enqueue_split(e);
+    if (split_schedule(e)) return enqueue_split(e, false);
     if (e->slab > 0 && e->R == 0) {
         int rc = enqueue_xy_slabs(e);
         if (rc) return rc;
@@ -752,6 +764,8 @@ int enqueue_diffusion(bfa_env* e) {
 }
 
 int enqueue_step(bfa_env* e) {
+    // the substrate-halves schedule runs each half's agents on its stream
+    if (e->hplanes.empty() && split_schedule(e)) return enqueue_split(e, true);
     int rc = enqueue_diffusion(e);
     if (rc) return rc;
     return launch_agents(e);

@@ -109,7 +109,8 @@ __device__ __forceinline__ double agent_segment(double rho, int d, int q, const 
     return rho;
 }
 
-// One agent step, one lane per (segment, substrate).  Two ways to store:
+// One agent step over substrates [d0, d0 + nd), one lane per (segment,
+// substrate).  Two ways to store:
 //  - LINES == false: each lane reads its voxel, applies its segment and
 //    stores the voxel (8 or 4 bytes);
 //  - LINES == true: the field is read and written in whole 128-byte lines.
@@ -132,19 +133,20 @@ __device__ __forceinline__ double agent_segment(double rho, int d, int q, const 
 constexpr int kLongSegment = 64;
 
 template <typename T, bool LINES>
-__global__ void k_agents(T* __restrict__ u, long long N, int ns, int nseg, int nag, const int* __restrict__ off,
-                         const long long* __restrict__ vox, const double* __restrict__ c1,
-                         const double* __restrict__ c2, const double* __restrict__ e2,
-                         const double* __restrict__ params, double* __restrict__ trk,
-                         const int* __restrict__ orig, const double* __restrict__ e1, double W) {
+__global__ void k_agents(T* __restrict__ u, long long N, int ns, int d0, int nd, int nseg, int nag,
+                         const int* __restrict__ off, const long long* __restrict__ vox,
+                         const double* __restrict__ c1, const double* __restrict__ c2,
+                         const double* __restrict__ e2, const double* __restrict__ params,
+                         double* __restrict__ trk, const int* __restrict__ orig, const double* __restrict__ e1,
+                         double W) {
     constexpr bool DEV = sizeof(T) == 4;
     constexpr int E = 128 / (int)sizeof(T);  // elements per line
     using VA = typename Vec16A<T>::type;
     constexpr int VW = 16 / (int)sizeof(T);
     const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= (long long)nseg * ns) return;
-    const int d = (int)(gid / nseg);
-    const int q = (int)(gid - (long long)d * nseg);
+    if (gid >= (long long)nseg * nd) return;
+    const int d = d0 + (int)(gid / nseg);
+    const int q = (int)(gid - (long long)(d - d0) * nseg);
     const long long lo = (long long)d * N;  // substrate d's elements: [lo, lo + N)
     double b = 0.0, s = 1.0, rs = 1.0;
     if (DEV) {
@@ -217,9 +219,11 @@ __global__ void k_agents(T* __restrict__ u, long long N, int ns, int nseg, int n
 // holding the segment's last agent stores the voxel.  The lanes' starting
 // densities come from composed maps, so the result agrees with the serial
 // walk to rounding (a few ulps per lane boundary), not bit for bit.  long_seg
-// lists the long segments.  Launch: 64 * nlong * ns threads.
+// lists the long segments.  Launch: 64 * nlong * nd threads for substrates
+// [d0, d0 + nd).
 template <typename T>
-__global__ __launch_bounds__(256) void k_agents_long(T* __restrict__ u, long long N, int ns, int nlong,
+__global__ __launch_bounds__(256) void k_agents_long(T* __restrict__ u, long long N, int ns, int d0, int nd,
+                                                     int nlong,
                                                      const int* __restrict__ long_seg, int nag,
                                                      const int* __restrict__ off, const long long* __restrict__ vox,
                                                      const double* __restrict__ c1, const double* __restrict__ c2,
@@ -230,9 +234,9 @@ __global__ __launch_bounds__(256) void k_agents_long(T* __restrict__ u, long lon
     constexpr bool DEV = sizeof(T) == 4;
     const long long wv = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
-    if (wv >= (long long)nlong * ns) return;  // whole wavefronts
-    const int d = (int)(wv / nlong);
-    const int q = long_seg[wv - (long long)d * nlong];
+    if (wv >= (long long)nlong * nd) return;  // whole wavefronts
+    const int d = d0 + (int)(wv / nlong);
+    const int q = long_seg[wv - (long long)(d - d0) * nlong];
     const int a0 = off[q], a1 = off[q + 1];
     const int K = (a1 - a0 + 63) / 64;
     const int b0 = min(a1, a0 + lane * K), b1 = min(a1, b0 + K);

@@ -217,30 +217,46 @@ def test_autotune_picks_the_fastest_shapes_grid_runs_and_schedule(monkeypatch):
 
 
 class FakeAgentsEnv:
-    """Step time 1.00 ms with whole-line agent stores, 0.97 with single voxels."""
+    """Step time 1.00 ms with whole-line agent stores, 0.97 with single
+    voxels; 2 % less on two substrate streams with single voxels only."""
+
+    ns = 4
 
     def __init__(self, clock):
         self.clock = clock
         self.lines = True
+        self.split = 1
         self.steps = 0
 
     def set_agent_stores(self, lines):
         self.lines = bool(lines)
 
+    def set_substrate_streams(self, g):
+        self.split = g
+
+    def substrate_streams(self):
+        return self.split
+
+    def slab(self):
+        return 0
+
     def step(self, dt, n):
-        self.clock.t += (1.0 if self.lines else 0.97) * n * 1e-3
+        t = 1.0 if self.lines else (0.97 * (0.98 if self.split == 2 else 1.0))
+        self.clock.t += t * n * 1e-3
         self.steps += n
 
     def synchronize(self):
         pass
 
 
-def test_autotune_agents_keeps_the_faster_store_form(monkeypatch):
+def test_autotune_agents_keeps_the_fastest_store_form_and_schedule(monkeypatch):
     at = load_autotune()
     clock = Clock()
     monkeypatch.setattr(time, "perf_counter", clock)
     env = FakeAgentsEnv(clock)
     res = at.autotune_agents(env, 0.01, steps=20, reps=2)
-    assert res["whole_lines"] is False and env.lines is False
-    assert res["ms_step"][True] == pytest.approx(1.0) and res["ms_step"][False] == pytest.approx(0.97)
-    assert env.steps == 2 * 2 * (20 + 4)
+    assert res["whole_lines"] is False and res["substrate_streams"] == 2
+    assert env.lines is False and env.split == 2
+    assert res["ms_step"][(True, 1)] == pytest.approx(1.0)
+    assert res["ms_step"][(False, 2)] == pytest.approx(0.97 * 0.98)
+    assert env.steps == 2 * 4 * (20 + 4)

@@ -4,7 +4,8 @@ same field and internalised totals, in FP64 and FP32, with agents piled onto
 shared voxels and lines, on a grid whose substrate blocks are not a whole
 number of lines (so the first and last lines of a substrate are shared with
 its neighbours), with agents on the first and last voxels, through bfa_step
-and bfa_secretion_uptake."""
+and bfa_secretion_uptake, and with each substrate half's agent step on its
+own stream."""
 import numpy as np
 import pytest
 
@@ -29,9 +30,10 @@ def problem():
     return p
 
 
-def run(pkg, p, precision, lines, eager=False):
+def run(pkg, p, precision, lines, eager=False, split=1):
     env = gpu_env(pkg, p, precision)
     try:
+        env.set_substrate_streams(split)
         env.set_agent_stores(lines)
         assert env.agent_stores() == lines
         env.track_internalized(np.linspace(0, 1, p.voxel.size * p.ns), p.e1)
@@ -52,6 +54,19 @@ def test_whole_line_and_single_voxel_stores_agree(pkg, precision, eager):
     p = problem()
     a, ta = run(pkg, p, precision, True, eager)
     b, tb = run(pkg, p, precision, False, eager)
+    assert np.array_equal(a, b)
+    assert np.array_equal(ta, tb)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("lines", [False, True])
+def test_agents_per_substrate_half_agree(pkg, precision, lines):
+    """On two substrate streams each half's agent step follows its own z
+    sweep (and overlaps the other half's sweeps); the result equals one
+    agent step after all sweeps, with substrate blocks sharing lines."""
+    p = problem()
+    a, ta = run(pkg, p, precision, lines, split=2)
+    b, tb = run(pkg, p, precision, lines, split=1)
     assert np.array_equal(a, b)
     assert np.array_equal(ta, tb)
 
