@@ -277,7 +277,8 @@ int bfa_get_slab(const bfa_env *env);
 /* Sweep schedule across substrates: groups == 2 sweeps the first and the
  * second half of the substrates (x, y, z each) on two parallel streams, so
  * that the tail of each kernel overlaps the other half's work; groups == 1
- * (default) sweeps all substrates with one launch per axis.  Per-line
+ * (default) sweeps all substrates with one launch per axis.  In bfa_step
+ * each half's agent step follows its own z sweep on its stream.  Per-line
  * arithmetic is the same, so the result is bit-identical.  Ignored while a
  * slab schedule is set.  bfa_time_kernels always times the single-group
  * sweeps kernel by kernel; bfa_step uses the schedule.  No reference
