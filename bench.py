@@ -637,6 +637,7 @@ def main():
                 for t in r["trials"]:
                     print(prec, json.dumps(t), file=sys.stderr)
     copy_gbs = 2 * 2 ** 30 / (pkg.copy_kernel_ms(2 ** 30, 10) * 1e-3) / 1e9
+    copy_stream_gbs = 2 * 2 ** 30 / (pkg.copy_stream_ms(2 ** 30, 10) * 1e-3) / 1e9
 
     tune_ag = not args.no_tune
     env32 = make_env(pkg, w, n, ns, "fp32", tiles["fp32"], tune_agents=tune_ag and not args.tiles)
@@ -703,6 +704,7 @@ def main():
         "kernel_ms": dict(x=ms32[0], y=ms32[1], z=ms32[2], agents=ms32[3]),
         "step_alg_frac_of_peak": (updates * 6 * 4) / (wall32 / args.steps) / 1e9 / PEAK_HBM_GBS,
         "copy_kernel_GBps": copy_gbs,
+        "copy_stream_GBps": copy_stream_gbs,
         "eager_ms_per_step": 1e3 * gwall32 / args.steps,
         "tiles": dict(Lx=tiles32[0], Ly=tiles32[1], Lz=tiles32[2], rows_x=tiles32[3],
                       cols_y=tiles32[4], cols_z=tiles32[5], slab=slab32,

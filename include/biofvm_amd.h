@@ -249,13 +249,18 @@ int bfa_synchronize(bfa_env *env);
  * returns the mean milliseconds per launch in ms[5] = {x sweep, y sweep,
  * z sweep, agents, whole step}.  bfa_copy_kernel times a plain device copy of
  * `bytes` bytes `reps` times (for calibrating HBM byte counters) and returns
- * the mean milliseconds per launch. */
+ * the mean milliseconds per launch; bfa_copy_kernel_stream does the same with
+ * the fastest copy form measured here (one pass, four 16-byte vectors per
+ * thread, nontemporal loads and stores; bytes <= 2^40), the achievable-
+ * bandwidth reference for the sweeps.  Neither has a counterpart in the
+ * reference. */
 int bfa_time_kernels(bfa_env *env, double dt, int reps, double *ms);
 int bfa_copy_kernel(size_t bytes, int reps, double *ms);
+int bfa_copy_kernel_stream(size_t bytes, int reps, double *ms);
 
 /* Kernel shape selection, used by the autotuner: the chunk length L of the
- * in-register line solve per axis (4, 8, 16, 32 or 64, and 25 for y and z;
- * 0 = automatic); the
+ * in-register line solve per axis (4, 8, 16, 32 or 64, 25 for y and z, 20
+ * and 28 for the wave-local x sweep, which takes at most 32; 0 = automatic); the
  * x-sweep form: rows_x == 0 selects the wave-local x sweep (a row's chunks
  * live in one wavefront), rows_x in {1,2,4,...,64} the LDS-tiled x sweep
  * with that many rows per tile; cols_y / cols_z, the x-columns per y / z

@@ -87,6 +87,7 @@ def lib():
     L.bfa_synchronize.argtypes = [_vp]
     L.bfa_time_kernels.argtypes = [_vp, ctypes.c_double, ctypes.c_int, _dp]
     L.bfa_copy_kernel.argtypes = [ctypes.c_size_t, ctypes.c_int, _dp]
+    L.bfa_copy_kernel_stream.argtypes = [ctypes.c_size_t, ctypes.c_int, _dp]
     L.bfa_set_tiles.argtypes = [_vp] + [ctypes.c_int] * 6
     L.bfa_get_tiles.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
     L.bfa_set_slab.argtypes = [_vp, ctypes.c_int]
@@ -166,6 +167,16 @@ def copy_kernel_ms(nbytes: int, reps: int = 20) -> float:
     rc = lib().bfa_copy_kernel(int(nbytes), int(reps), ctypes.byref(out))
     if rc:
         raise BioFVMError(f"bfa_copy_kernel failed with status {rc}")
+    return out.value
+
+
+def copy_stream_ms(nbytes: int, reps: int = 20) -> float:
+    """Mean time of the one-pass streaming device copy of nbytes (the
+    achievable-bandwidth reference)."""
+    out = ctypes.c_double()
+    rc = lib().bfa_copy_kernel_stream(int(nbytes), int(reps), ctypes.byref(out))
+    if rc:
+        raise BioFVMError(f"bfa_copy_kernel_stream failed with status {rc}")
     return out.value
 
 

@@ -4,8 +4,9 @@ For one grid shape and precision, times every admissible shape of the three
 sweep kernels on the GPU (HIP events around each launch, via
 ``Microenvironment.time_kernels``) and returns the fastest per axis:
 
-* x sweep: the wave-local form with chunk length L in {4, 8, 16, 32}
-  (a row's ceil(nx/L) <= 64 chunks share one wavefront), or the LDS-tiled
+* x sweep: the wave-local form with chunk length L in {4, 8, 16, 32}, and
+  in FP64 also 20 and 28 (a row's ceil(nx/L) <= 64 chunks share one
+  wavefront), or the LDS-tiled
   form with R rows per tile and chunk length L; for the wave-local form, the
   number of resident blocks per compute unit (fewer waves in flight can
   stream faster);
@@ -29,7 +30,12 @@ import os
 
 import numpy as np
 
-X_WAVE_L = (4, 8, 16, 32)
+# Wave-local x chunk lengths per precision.  FP64 takes 20 and 28 too: at
+# 400^3 x 4 with the k-slab schedule its step went from 2.39 to 2.20 ms with
+# 20-node chunks.  FP32 does not: its x sweep timed alone is no faster with
+# them (0.393 against 0.394 ms), and in the step with 100k agents it is
+# slower (1.27-1.33 against 1.22 ms), which a search without agents misses.
+X_WAVE_L = {"fp32": (4, 8, 16, 32), "fp64": (4, 8, 16, 20, 28, 32)}
 X_LDS = [(R, L) for R in (8, 16, 32) for L in (8, 16, 32)]
 COL_TX = (16, 32, 64)
 COL_L = (4, 8, 16, 25, 32)
@@ -94,7 +100,7 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=6, cache_path=
         # wave-local x sweep: chunk length and resident blocks per CU together
         # (the best block count differs between chunk lengths)
         best_x = None
-        for L in X_WAVE_L:
+        for L in X_WAVE_L["fp32" if precision in ("fp32", "f32", "float", 1) else "fp64"]:
             env.set_x_grid(0)
             ms = _time(env, dt, reps, (L, 0, 0, 0, 0, 0))
             if ms is None:

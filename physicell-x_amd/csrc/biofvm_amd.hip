@@ -32,6 +32,11 @@ constexpr int kMaxSubstrates = 64;
 constexpr int kLs[] = {4, 8, 16, 32, 64};
 // column sweeps also take 25-node chunks (16 chunks of a 400-node line)
 constexpr int kColLs[] = {4, 8, 16, 25, 32, 64};
+// the wave-local x sweep also takes 20 and 28 (whole 16-byte vectors in FP32
+// and FP64): 20 chunks of a 400-node row put three rows in a wavefront
+// (60 of 64 lanes, against one row of 50 lanes with 8-node chunks), 29 chunks
+// of an 800-node row two (58 lanes)
+constexpr int kXWaveLs[] = {4, 8, 16, 20, 28, 32};
 
 }  // namespace
 
@@ -260,7 +265,7 @@ int choose_tiles(bfa_env* e) {
             L = e->prec == BFA_FP32 ? 16 : 8;
             while (L < 64 && (nx + L - 1) / L > 64) L <<= 1;
         }
-        if (std::find(std::begin(kLs), std::end(kLs), L) == std::end(kLs) || L > 32)
+        if (std::find(std::begin(kXWaveLs), std::end(kXWaveLs), L) == std::end(kXWaveLs))
             return fail(e, BFA_ERR_ARG, "wave x sweep: unsupported chunk length %d", L);
         if ((nx + L - 1) / L > 64)
             return fail(e, BFA_ERR_ARG, "wave x sweep: nx=%d needs more than 64 chunks of %d", nx, L);
@@ -461,7 +466,7 @@ int raise_lds_limits(bfa_env* e) {
         fw = (const void*)bfa::k_sweep_x_wave<T, Lv, DEV, false>;             \
         fwr = (const void*)bfa::k_sweep_x_wave<T, Lv, DEV, true>;             \
     }
-        BFA_XW(4) BFA_XW(8) BFA_XW(16) BFA_XW(32)
+        BFA_XW(4) BFA_XW(8) BFA_XW(16) BFA_XW(20) BFA_XW(28) BFA_XW(32)
 #undef BFA_XW
         e->xw_occ = fw ? occupancy_blocks(fw, 256) : 1;
         e->xw_occ_fr = fwr ? occupancy_blocks(fwr, 256) : 1;
@@ -477,6 +482,8 @@ int launch_sweep(bfa_env* e, int axis, hipStream_t st, const Win& w) {
             case 4: return launch_xw_L<T, 4>(e, st, w);
             case 8: return launch_xw_L<T, 8>(e, st, w);
             case 16: return launch_xw_L<T, 16>(e, st, w);
+            case 20: return launch_xw_L<T, 20>(e, st, w);
+            case 28: return launch_xw_L<T, 28>(e, st, w);
             case 32: return launch_xw_L<T, 32>(e, st, w);
         }
     } else if (axis == 0) {
@@ -1866,7 +1873,9 @@ int bfa_set_agent_stores(bfa_env* e, int whole_lines) {
 
 int bfa_get_agent_stores(const bfa_env* e) { return e ? (e->agent_lines ? 1 : 0) : -1; }
 
-int bfa_copy_kernel(size_t bytes, int reps, double* ms) {
+// Mean milliseconds of `reps` device copies of `bytes` bytes (after one
+// untimed launch): the grid-stride k_copy16 or the one-pass k_copy_stream.
+static int time_copy(size_t bytes, int reps, double* ms, bool stream) {
     if (!ms || reps < 1 || bytes < 16) return BFA_ERR_ARG;
     const long long n = (long long)(bytes / 16);
     uint4 *a = nullptr, *b = nullptr;
@@ -1879,11 +1888,18 @@ int bfa_copy_kernel(size_t bytes, int reps, double* ms) {
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    const unsigned blocks = (unsigned)std::min<long long>(65536, (n + 255) / 256);
-    hipLaunchKernelGGL(bfa::k_copy16, dim3(blocks), dim3(256), 0, nullptr, a, b, n);
+    const unsigned blocks = stream ? (unsigned)((n + 1023) / 1024)
+                                   : (unsigned)std::min<long long>(65536, (n + 255) / 256);
+    auto launch = [&] {
+        if (stream)
+            hipLaunchKernelGGL(bfa::k_copy_stream, dim3(blocks), dim3(256), 0, nullptr,
+                               reinterpret_cast<const bfa::u32x4*>(a), reinterpret_cast<bfa::u32x4*>(b), n);
+        else
+            hipLaunchKernelGGL(bfa::k_copy16, dim3(blocks), dim3(256), 0, nullptr, a, b, n);
+    };
+    launch();
     (void)hipEventRecord(e0, nullptr);
-    for (int r = 0; r < reps; ++r)
-        hipLaunchKernelGGL(bfa::k_copy16, dim3(blocks), dim3(256), 0, nullptr, a, b, n);
+    for (int r = 0; r < reps; ++r) launch();
     (void)hipEventRecord(e1, nullptr);
     hipError_t st = hipEventSynchronize(e1);
     float t = 0;
@@ -1895,6 +1911,13 @@ int bfa_copy_kernel(size_t bytes, int reps, double* ms) {
     if (st != hipSuccess) return BFA_ERR_HIP;
     *ms = t / reps;
     return BFA_OK;
+}
+
+int bfa_copy_kernel(size_t bytes, int reps, double* ms) { return time_copy(bytes, reps, ms, false); }
+
+int bfa_copy_kernel_stream(size_t bytes, int reps, double* ms) {
+    if (bytes > (size_t)1 << 40) return BFA_ERR_ARG;  // grid.x stays below 2^31
+    return time_copy(bytes, reps, ms, true);
 }
 
 int bfa_set_tiles(bfa_env* e, int Lx, int Ly, int Lz, int rows_x, int cols_y, int cols_z) {

