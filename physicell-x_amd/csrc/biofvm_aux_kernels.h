@@ -643,8 +643,8 @@ __global__ void k_copy16(const uint4* __restrict__ in, uint4* __restrict__ out, 
 // One-pass streaming copy, the achievable-bandwidth reference: the grid
 // covers the buffer once, each thread moving four 16-byte vectors (a block's
 // 256 threads apart) with nontemporal loads and stores.  Measured on MI355X
-// at 1 GiB: 6.2-6.3 TB/s, against 5.4 TB/s for k_copy16's grid-stride loop
-// over 65536 blocks.
+// at 1 GiB: 6.0-6.3 TB/s, against 5.4-5.6 TB/s for k_copy16's grid-stride
+// loop over 65536 blocks.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void k_copy_stream(const u32x4* __restrict__ in,
                                                      u32x4* __restrict__ out, long long n) {
