@@ -154,24 +154,26 @@ def run_host_boundary(env, dt, updates, prec, reps=3, fix_basis=False):
             "pcie_bytes_down": int(host.size * (4 if fp32 else 8)), "reps": reps}
 
 
-def load_traffic(kernel_key, tiles, runs=(1, 1)):
-    """HBM bytes per launch of a sweep kernel, from the counter runs recorded
-    in profiles/traffic.json, or None unless an entry was measured with this
-    exact kernel shape (tiles = Lx, Ly, Lz, rows_x, cols_y, cols_z) and tile
-    order (runs = y and z tile sub-runs per XCD): another shape moves other
-    bytes, so its figure is not reported."""
+def load_traffic(kernel_key, tiles):
+    """(HBM bytes per launch, y and z tile sub-runs of the counter run) of a
+    sweep kernel, from the counter runs recorded in profiles/traffic.json, or
+    (None, None) unless an entry was measured with this exact kernel shape
+    (tiles = Lx, Ly, Lz, rows_x, cols_y, cols_z): another shape moves other
+    bytes, so its figure is not reported.  The tile order (sub-runs per XCD)
+    only changes which block solves which tile, and the counters read the
+    same bytes for every order measured (z at 400^3 and 800^3: 1.000x at one
+    run and at the tuned count), so an entry of another order is reported,
+    with its order."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
-        return None
+        return None, None
     try:
         ent = json.load(open(path)).get("entries", {}).get(kernel_key)
     except Exception:
-        return None
+        return None, None
     if not ent or [int(t) for t in ent.get("tiles", [])] != [int(t) for t in list(tiles)[:6]]:
-        return None
-    if [int(r) for r in ent.get("runs", [1, 1])] != [int(r) for r in runs]:
-        return None
-    return ent.get("bytes")
+        return None, None
+    return ent.get("bytes"), [int(r) for r in ent.get("runs", [1, 1])]
 
 
 def parse_tiles(spec):
@@ -550,14 +552,14 @@ def run_large(pkg, n=800, steps=20, seed=3, tune=True, fixed=None):
         dom = int(np.argmax(ms[:3]))
         alg = updates * 2 * tsize
         achieved = alg / (ms[dom] * 1e-3) / 1e9
+        traf = load_traffic(f"sweep_{names[dom]}_{prec}_{n}", e.tiles())
         out[prec] = {"value": updates * (steps - 1) / dt_, "ms_per_step": 1e3 * dt_ / (steps - 1),
                      "step_alg_frac_of_peak": updates * 6 * tsize / (dt_ / (steps - 1)) / 1e9 / PEAK_HBM_GBS,
                      "kernel_ms": dict(x=ms[0], y=ms[1], z=ms[2], agents=ms[3]),
                      "roofline": {"bound": "hbm", "kernel": f"sweep_{names[dom]}", "alg_bytes_per_launch": alg,
                                   "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                                   "frac": achieved / PEAK_HBM_GBS,
-                                  "traffic": load_traffic(f"sweep_{names[dom]}_{prec}_{n}", e.tiles(),
-                                                          e.col_runs())},
+                                  "traffic": traf[0], "traffic_col_runs": traf[1]},
                      "tiles": list(e.tiles()), "autotuned": tune and (fixed or {}).get(prec) is None,
                      "schedule": {"slab": e.slab(), "substrate_streams": e.substrate_streams(),
                                   "x_grid": e.x_grid()[0], "runs_y": e.col_runs()[0], "runs_z": e.col_runs()[1]},
@@ -680,7 +682,7 @@ def main():
     sweeps = 2 if (slab32 and dom == 0) else 1
     alg_bytes = updates * 2 * 4 * sweeps  # one FP32 read + one write per voxel-substrate per sweep
     achieved = alg_bytes / (ms32[dom] * 1e-3) / 1e9
-    traffic = load_traffic(f"sweep_{names[dom]}_fp32_{n}", tiles32, runs32)
+    traffic, traffic_runs = load_traffic(f"sweep_{names[dom]}_fp32_{n}", tiles32)
     out = {
         "metric": METRIC,
         "value": updates * args.steps / wall32,
@@ -700,7 +702,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": f"sweep_{names[dom]}", "sweeps_in_kernel": sweeps,
                      "alg_bytes_per_launch": alg_bytes, "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": traffic},
+                     "traffic": traffic, "traffic_col_runs": traffic_runs},
         "kernel_ms": dict(x=ms32[0], y=ms32[1], z=ms32[2], agents=ms32[3]),
         "step_alg_frac_of_peak": (updates * 6 * 4) / (wall32 / args.steps) / 1e9 / PEAK_HBM_GBS,
         "copy_kernel_GBps": copy_gbs,
@@ -710,8 +712,7 @@ def main():
                       cols_y=tiles32[4], cols_z=tiles32[5], slab=slab32,
                       substrate_streams=split32,
                       x_grid=(tiles["fp32"] or {}).get("x_grid", 0),
-                      runs_y=(tiles["fp32"] or {}).get("runs_y", 1),
-                      runs_z=(tiles["fp32"] or {}).get("runs_z", 1),
+                      runs_y=runs32[0], runs_z=runs32[1],
                       agent_stores="whole lines" if lines32 else "single voxels",
                       autotuned=not args.no_tune and not args.tiles),
         "aux_kernels": aux,
