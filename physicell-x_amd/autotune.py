@@ -17,8 +17,9 @@ sweep kernels on the GPU (HIP events around each launch, via
 * y and z sweeps at the chosen shapes: how many sub-runs of tiles each XCD
   advances together (``Microenvironment.set_col_runs``), per axis;
 * schedule: whole-volume sweeps, the two substrate halves swept on parallel
-  streams, or x/y sweeps in k-slabs of a few planes (chosen by graph-replayed
-  whole-step time).
+  streams, or x/y sweeps in k-slabs of a few planes, the slabs with each of
+  the three fastest wave x chunk lengths (chosen by graph-replayed whole-step
+  time).
 
 Results are cached per (nx, ny, nz, ns, precision) in memory and, when a path
 is given, in a JSON file.
@@ -42,6 +43,7 @@ COL_L = (4, 8, 16, 25, 32)
 SLAB_BYTES = (8 << 20, 16 << 20, 32 << 20, 64 << 20, 128 << 20)
 X_GRID = (2, 3, 4, 5, 6, 8)
 COL_RUNS = (1, 2, 3, 4, 5, 6, 8, 12)
+SLAB_X_SHAPES = 3
 
 _cache: dict = {}
 
@@ -100,12 +102,14 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=6, cache_path=
         # wave-local x sweep: chunk length and resident blocks per CU together
         # (the best block count differs between chunk lengths)
         best_x = None
+        wave_best = {}  # chunk length -> (ms, blocks per CU) of its best grid
         for L in X_WAVE_L["fp32" if precision in ("fp32", "f32", "float", 1) else "fp64"]:
             env.set_x_grid(0)
             ms = _time(env, dt, reps, (L, 0, 0, 0, 0, 0))
             if ms is None:
                 continue
             trials.append({"axis": "x", "form": "wave", "L": L, "blocks_per_cu": 0, "ms": ms[0]})
+            wave_best[L] = (ms[0], 0)
             if best_x is None or ms[0] < best_x[0]:
                 best_x = (ms[0], L, 0, 0)
             resident = env.x_grid()[1]
@@ -116,6 +120,8 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=6, cache_path=
                 env.time_kernels(dt, 1)
                 ms = env.time_kernels(dt, reps)
                 trials.append({"axis": "x", "form": "wave", "L": L, "blocks_per_cu": g, "ms": ms[0]})
+                if ms[0] < wave_best[L][0]:
+                    wave_best[L] = (ms[0], g)
                 if ms[0] < best_x[0]:
                     best_x = (ms[0], L, 0, g)
         env.set_x_grid(0)
@@ -155,25 +161,36 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=6, cache_path=
                 best_rz = (ms[2], R)
         env.set_col_runs(best_ry[1], best_rz[1])
         # schedules, by graph-replayed step time: whole-volume sweeps, the two
-        # substrate halves on parallel streams, x/y k-slabs
-        cands = [(0, 1)]
+        # substrate halves on parallel streams, x/y k-slabs.  The k-slab
+        # schedule runs the x sweep of one slab beside the y sweep of the one
+        # before, and the x chunk length that is fastest alone is not always
+        # the fastest there (FP64 at 800^3 x 2: 28-node chunks in slabs of 25
+        # planes 9.29 ms per step, against 9.56 with the 32-node chunks that
+        # win alone), so slabs are timed with the SLAB_X_SHAPES fastest chunk
+        # lengths, each at its best grid.
+        cands = [(0, 1, best_x[1], best_x[3])]
         if ns > 1:
-            cands.append((0, 2))
+            cands.append((0, 2, best_x[1], best_x[3]))
         if best_x[2] == 0:
             plane = nx * ny * (4 if precision in ("fp32", "f32", 1) else 8)
-            cands += [(max(1, nb // plane), 1) for nb in SLAB_BYTES if max(1, nb // plane) < nz]
+            xs = sorted(wave_best.items(), key=lambda kv: kv[1][0])[:SLAB_X_SHAPES]
+            cands += [(max(1, nb // plane), 1, L, g) for nb in SLAB_BYTES if max(1, nb // plane) < nz
+                      for L, (_, g) in xs]
         best_sched = None
-        for slab, split in cands:
+        for slab, split, L, g in cands:
+            env.set_tiles(L, *shape[1:])
+            env.set_x_grid(g)
             env.set_slab(slab)
             env.set_substrate_streams(split)
             t = _step_ms(env, dt, reps)
-            trials.append({"axis": "schedule", "slab": slab, "substrate_streams": split, "ms_step": t})
+            trials.append({"axis": "schedule", "slab": slab, "substrate_streams": split, "Lx": L,
+                           "blocks_per_cu": g, "ms_step": t})
             if best_sched is None or t < best_sched[0]:
-                best_sched = (t, slab, split)
+                best_sched = (t, slab, split, L, g)
         env.set_slab(0)
         env.set_substrate_streams(1)
-        res = {"Lx": best_x[1], "rows_x": best_x[2], "Ly": best_y[1], "cols_y": best_y[2],
-               "Lz": best_z[1], "cols_z": best_z[2], "x_grid": best_grid[1],
+        res = {"Lx": best_sched[3], "rows_x": best_x[2], "Ly": best_y[1], "cols_y": best_y[2],
+               "Lz": best_z[1], "cols_z": best_z[2], "x_grid": best_sched[4],
                "runs_y": best_ry[1], "runs_z": best_rz[1], "slab": best_sched[1],
                "substrate_streams": best_sched[2],
                "ms": {"x": best_grid[0], "y": best_ry[0], "z": best_rz[0], "step": best_sched[0]},

@@ -216,6 +216,44 @@ def test_autotune_picks_the_fastest_shapes_grid_runs_and_schedule(monkeypatch):
     assert e2.shape == (8, 25, 25, 0, 32, 32) and e2.grid == 4 and e2.runs == (1, 3) and e2.split == 2
 
 
+class FakeSlabEnv(FakeEnv):
+    """FakeEnv whose k-slab schedule runs 10 % faster with x chunks of 16 at
+    four blocks per CU, a shape that is not the fastest x sweep alone."""
+
+    log = []
+
+    def __init__(self, *a):
+        super().__init__(*a)
+        FakeSlabEnv.log.append(self)
+
+    def step(self, dt, n):
+        t = sum(self.kernel_ms()) * (0.98 if self.split == 2 else 1.0)
+        if self.slab:
+            t *= 0.9 if (self.shape[0], self.grid) == (16, 4) else 1.01
+        self.clock.t += t * n * 1e-3
+
+
+def test_autotune_times_slabs_with_other_x_chunk_lengths(monkeypatch):
+    at = load_autotune()
+    clock = Clock()
+    monkeypatch.setattr(time, "perf_counter", clock)
+    FakeEnv.clock = clock
+    FakeSlabEnv.log = []
+
+    class Pkg:
+        Microenvironment = FakeSlabEnv
+
+    res = at.autotune(Pkg, 400, 400, 400, 4, "fp32", 0.01)
+    # the x sweep alone is fastest at chunk 8 (grid 4); the slab schedule with
+    # chunk 16 at its own best grid (4) wins the whole step
+    assert res["slab"] > 0 and res["substrate_streams"] == 1
+    assert (res["Lx"], res["x_grid"], res["rows_x"]) == (16, 4, 0)
+    assert (res["Ly"], res["cols_y"], res["Lz"], res["cols_z"]) == (25, 32, 25, 32)
+    e2 = FakeEnv(400, 400, 400, 4, 20.0, "fp32")
+    at.apply(e2, res)
+    assert e2.shape == (16, 25, 25, 0, 32, 32) and e2.grid == 4 and e2.slab == res["slab"]
+
+
 class FakeAgentsEnv:
     """Step time 1.00 ms with whole-line agent stores, 0.97 with single
     voxels; 2 % less on two substrate streams with single voxels only."""
