@@ -221,6 +221,7 @@ class FakeSlabEnv(FakeEnv):
     four blocks per CU, a shape that is not the fastest x sweep alone."""
 
     log = []
+    gain = 0.9
 
     def __init__(self, *a):
         super().__init__(*a)
@@ -229,7 +230,7 @@ class FakeSlabEnv(FakeEnv):
     def step(self, dt, n):
         t = sum(self.kernel_ms()) * (0.98 if self.split == 2 else 1.0)
         if self.slab:
-            t *= 0.9 if (self.shape[0], self.grid) == (16, 4) else 1.01
+            t *= self.gain if (self.shape[0], self.grid) == (16, 4) else 1.01
         self.clock.t += t * n * 1e-3
 
 
@@ -252,6 +253,29 @@ def test_autotune_times_slabs_with_other_x_chunk_lengths(monkeypatch):
     e2 = FakeEnv(400, 400, 400, 4, 20.0, "fp32")
     at.apply(e2, res)
     assert e2.shape == (16, 25, 25, 0, 32, 32) and e2.grid == 4 and e2.slab == res["slab"]
+
+
+def test_autotune_keeps_whole_volume_sweeps_unless_slabs_win_clearly(monkeypatch):
+    """A k-slab schedule 1 % faster than the best whole-volume one is not
+    taken: it has to win by the margin.  Whole volume: kernels 1.22 ms x 0.98
+    (two substrate groups) = 1.1956; slabs with 16-node x chunks: 1.24 x 0.955
+    = 1.1842, 0.990 of it."""
+    at = load_autotune()
+    clock = Clock()
+    monkeypatch.setattr(time, "perf_counter", clock)
+    FakeEnv.clock = clock
+
+    class Near(FakeSlabEnv):
+        gain = 0.955
+
+    class Pkg:
+        Microenvironment = Near
+
+    res = at.autotune(Pkg, 400, 400, 400, 4, "fp32", 0.01)
+    assert (res["slab"], res["substrate_streams"]) == (0, 2)
+    assert (res["Lx"], res["x_grid"]) == (8, 4)
+    slabs = [t["ms_step"] for t in res["trials"] if t.get("axis") == "schedule" and t["slab"]]
+    assert min(slabs) == pytest.approx(0.990 * res["ms"]["step"], rel=1e-3)
 
 
 class FakeAgentsEnv:
