@@ -17,9 +17,9 @@ sweep kernels on the GPU (HIP events around each launch, via
 * y and z sweeps at the chosen shapes: how many sub-runs of tiles each XCD
   advances together (``Microenvironment.set_col_runs``), per axis;
 * schedule: whole-volume sweeps, the two substrate halves swept on parallel
-  streams, or x/y sweeps in k-slabs of a few planes, the slabs with each of
-  the three fastest wave x chunk lengths (chosen by graph-replayed whole-step
-  time).
+  streams, or (FP64) x/y sweeps in k-slabs of a few planes, the slabs with
+  each of the three fastest wave x chunk lengths (chosen by graph-replayed
+  whole-step time; slabs have to win by 2 %).
 
 Results are cached per (nx, ny, nz, ns, precision) in memory and, when a path
 is given, in a JSON file.
@@ -172,8 +172,14 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=6, cache_path=
         cands = [(0, 1, best_x[1], best_x[3])]
         if ns > 1:
             cands.append((0, 2, best_x[1], best_x[3]))
-        if best_x[2] == 0:
-            plane = nx * ny * (4 if precision in ("fp32", "f32", 1) else 8)
+        # FP32 keeps whole-volume sweeps: on three boxes its k-slab schedule
+        # came out between 2 % slower (alternating runs with 100k agents) and
+        # 2 % faster (this search, without agents) at 400^3 x 4, never clearly
+        # ahead, while it costs the single launch per axis whose counters
+        # and roofline the rest of the measurement rests on
+        fp32 = precision in ("fp32", "f32", "float", 1)
+        if best_x[2] == 0 and not fp32:
+            plane = nx * ny * 8
             xs = sorted(wave_best.items(), key=lambda kv: kv[1][0])[:SLAB_X_SHAPES]
             cands += [(max(1, nb // plane), 1, L, g) for nb in SLAB_BYTES if max(1, nb // plane) < nz
                       for L, (_, g) in xs]

@@ -199,6 +199,7 @@ def test_autotune_picks_the_fastest_shapes_grid_runs_and_schedule(monkeypatch):
     assert (res["Ly"], res["cols_y"], res["Lz"], res["cols_z"]) == (25, 32, 25, 32)
     assert (res["runs_y"], res["runs_z"]) == (1, 3)
     assert (res["slab"], res["substrate_streams"]) == (0, 2)
+    assert not [t for t in res["trials"] if t.get("axis") == "schedule" and t["slab"]]  # FP32: none
     assert res["ms"]["z"] == pytest.approx(0.42) and res["ms"]["y"] == pytest.approx(0.37)
     env = FakeEnv.log[0]
     timed = env.timed
@@ -244,7 +245,7 @@ def test_autotune_times_slabs_with_other_x_chunk_lengths(monkeypatch):
     class Pkg:
         Microenvironment = FakeSlabEnv
 
-    res = at.autotune(Pkg, 400, 400, 400, 4, "fp32", 0.01)
+    res = at.autotune(Pkg, 400, 400, 400, 4, "fp64", 0.01)
     # the x sweep alone is fastest at chunk 8 (grid 4); the slab schedule with
     # chunk 16 at its own best grid (4) wins the whole step
     assert res["slab"] > 0 and res["substrate_streams"] == 1
@@ -271,7 +272,7 @@ def test_autotune_keeps_whole_volume_sweeps_unless_slabs_win_clearly(monkeypatch
     class Pkg:
         Microenvironment = Near
 
-    res = at.autotune(Pkg, 400, 400, 400, 4, "fp32", 0.01)
+    res = at.autotune(Pkg, 400, 400, 400, 4, "fp64", 0.01)
     assert (res["slab"], res["substrate_streams"]) == (0, 2)
     assert (res["Lx"], res["x_grid"]) == (8, 4)
     slabs = [t["ms_step"] for t in res["trials"] if t.get("axis") == "schedule" and t["slab"]]
