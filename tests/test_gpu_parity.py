@@ -472,14 +472,20 @@ def test_fp32_wave_x_chunk_lengths_match_oracle(pkg, nx):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("planes", [1, 3, 7, 40])
-def test_slab_schedule_is_bit_identical(pkg, precision, planes):
+@pytest.mark.parametrize("lx", [0, 20, 28])
+def test_slab_schedule_is_bit_identical(pkg, precision, planes, lx):
     """x/y sweeps in k-slabs on two streams give exactly the whole-volume result
-    (every line is solved by the same arithmetic either way), eager and graph."""
+    (every line is solved by the same arithmetic either way), eager and graph,
+    at the default x chunk and at the 20- and 28-node chunks the FP64 search
+    pairs with slabs."""
     p = make_problem(36, 29, 23, 3, seed=41, n_agents=200, dirichlet="mixed", export=5.0)
     ref = gpu_env(pkg, p, precision)
     sl = gpu_env(pkg, p, precision)
     eager = gpu_env(pkg, p, precision)
     try:
+        if lx:
+            for e in (ref, sl, eager):
+                e.set_tiles(lx, 0, 0, 0, 0, 0)
         sl.set_slab(planes)
         eager.set_slab(planes)
         assert sl.slab() == min(planes, p.nz)
