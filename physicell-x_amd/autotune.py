@@ -210,7 +210,8 @@ def autotune(pkg, nx, ny, nz, ns, precision="fp32", dt=0.01, reps=6, cache_path=
                "Lz": best_z[1], "cols_z": best_z[2], "x_grid": best_sched[4],
                "runs_y": best_ry[1], "runs_z": best_rz[1], "slab": best_sched[1],
                "substrate_streams": best_sched[2],
-               "ms": {"x": best_grid[0], "y": best_ry[0], "z": best_rz[0], "step": best_sched[0]},
+               "ms": {"x": wave_best[best_sched[3]][0] if best_sched[3] != best_x[1] else best_grid[0],
+                      "y": best_ry[0], "z": best_rz[0], "step": best_sched[0]},
                "trials": trials}
     finally:
         env.close()

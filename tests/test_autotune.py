@@ -250,6 +250,7 @@ def test_autotune_times_slabs_with_other_x_chunk_lengths(monkeypatch):
     # chunk 16 at its own best grid (4) wins the whole step
     assert res["slab"] > 0 and res["substrate_streams"] == 1
     assert (res["Lx"], res["x_grid"], res["rows_x"]) == (16, 4, 0)
+    assert res["ms"]["x"] == pytest.approx(0.42)  # the chosen chunk's own x time
     assert (res["Ly"], res["cols_y"], res["Lz"], res["cols_z"]) == (25, 32, 25, 32)
     e2 = FakeEnv(400, 400, 400, 4, 20.0, "fp32")
     at.apply(e2, res)
